@@ -1,0 +1,804 @@
+// MI355X (gfx950) kernels of one LULESH Lagrange leap-frog cycle, FP64.
+//
+// One cycle = five launches on the context stream:
+//   k_force  element-parallel: stress corner forces (integrateStressForElems)
+//            and Flanagan-Belytschko hourglass corner forces
+//            (calcHourglassControlForElems + calcFBHourglassForceForElems)
+//   k_node   node-parallel: fixed-order corner gather (stress sum, then
+//            hourglass sum, as the reference sums them), acceleration,
+//            symmetry-plane BCs, velocity, position
+//   k_kin    element-parallel: calcKinematicsForElems + the deviatoric loop
+//            + calcMonotonicQGradientsForElems
+//   k_qeos   element-parallel: calcMonotonicQRegionForElems, the QStop check,
+//            applyMaterialPropertiesForElems/evalEOSForElems,
+//            updateVolumesForElems and the per-element Courant/hydro terms,
+//            reduced to one minimum per block
+//   k_dt     one block: the final minima (calcTimeConstraintsForElems) and,
+//            in driver-loop mode, the loop test and the next timeIncrement!
+// No atomics touch field data: corners are written once and gathered in the
+// reference's order, so the FP64 result is deterministic run to run.
+#include "lulesh_elem.h"
+#include "lulesh_kernels.h"
+
+namespace lulesh {
+
+constexpr int BLOCK = 256;
+
+// ---------------------------------------------------------------------------
+// structured-mesh indexing (buildMesh!, src/domain.jl:117-174)
+// ---------------------------------------------------------------------------
+LDEV void elem_nodes(int64_t e, int s, int64_t (&nid)[8]) {
+  const int64_t s1 = s + 1, P = s1 * s1;
+  const int64_t col = e % s, rest = e / s;
+  const int64_t row = rest % s, plane = rest / s;
+  const int64_t n0 = plane * P + row * s1 + col;
+  nid[0] = n0;
+  nid[1] = n0 + 1;
+  nid[2] = n0 + s1 + 1;
+  nid[3] = n0 + s1;
+  nid[4] = n0 + P;
+  nid[5] = n0 + P + 1;
+  nid[6] = n0 + P + s1 + 1;
+  nid[7] = n0 + P + s1;
+}
+
+template <typename T>
+LDEV void gather8(const T *__restrict__ src, const int64_t (&nid)[8], T (&out)[8]) {
+#pragma unroll
+  for (int n = 0; n < 8; ++n) out[n] = src[nid[n]];
+}
+
+LDEV void raise(DevScalars *sc, int code) { atomicMin(&sc->err, code); }
+
+// timeIncrement!, src/domain.jl:920-961
+LDEV void time_increment(DevScalars *sc) {
+  double targetdt = sc->stoptime - sc->time;
+  if (sc->dtfixed <= 0.0 && sc->cycle != 0) {
+    const double olddt = sc->deltatime;
+    double gnewdt = __builtin_huge_val();
+    if (sc->dtcourant < gnewdt) gnewdt = sc->dtcourant / 2.0;
+    if (sc->dthydro < gnewdt) gnewdt = (sc->dthydro * 2.0) / 3.0;
+    double newdt = gnewdt;
+    const double ratio = newdt / olddt;
+    if (ratio >= 1.0) {
+      if (ratio < sc->deltatimemultlb) newdt = olddt;
+      else if (ratio > sc->deltatimemultub) newdt = olddt * sc->deltatimemultub;
+    }
+    newdt = jmin(newdt, sc->dtmax);
+    sc->deltatime = newdt;
+  }
+  if (sc->deltatime < targetdt && targetdt < (4.0 * sc->deltatime) / 3.0)
+    targetdt = (2.0 * sc->deltatime) / 3.0;
+  if (targetdt < sc->deltatime) sc->deltatime = targetdt;
+  sc->time = sc->time + sc->deltatime;
+  sc->cycle = sc->cycle + 1;
+}
+
+// ---------------------------------------------------------------------------
+// Domain(prob) on the device (src/domain.jl:117-174, 615-771, 822-916)
+// ---------------------------------------------------------------------------
+__global__ void k_init_nodes(int s, int64_t N, DevFields f) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const int64_t s1 = s + 1;
+  const int64_t col = n % s1, rest = n / s1;
+  const int64_t row = rest % s1, plane = rest / s1;
+  // tx = 1.125*(colLoc*nx + col)/meshEdgeElems with colLoc = 0, tp = 1
+  f.x[n] = (1.125 * (double)col) / (double)s;
+  f.y[n] = (1.125 * (double)row) / (double)s;
+  f.z[n] = (1.125 * (double)plane) / (double)s;
+  f.xd[n] = 0.0; f.yd[n] = 0.0; f.zd[n] = 0.0;
+  f.xdd[n] = 0.0; f.ydd[n] = 0.0; f.zdd[n] = 0.0;
+  f.fx[n] = 0.0; f.fy[n] = 0.0; f.fz[n] = 0.0;
+}
+
+__global__ void k_init_elems(int s, int64_t E, DevFields f) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int64_t nid[8];
+  elem_nodes(e, s, nid);
+  double X[8], Y[8], Z[8];
+  gather8(f.x, nid, X); gather8(f.y, nid, Y); gather8(f.z, nid, Z);
+  const double vol = elem_volume(X, Y, Z);
+  f.volo[e] = vol;
+  f.elemMass[e] = vol;
+  f.v[e] = 1.0;
+  f.e[e] = 0.0; f.p[e] = 0.0; f.q[e] = 0.0; f.ss[e] = 0.0;
+  f.ql[e] = 0.0; f.qq[e] = 0.0; f.delv[e] = 0.0; f.vdov[e] = 0.0; f.arealg[e] = 0.0;
+  f.vnew[e] = 0.0; f.dxx[e] = 0.0; f.dyy[e] = 0.0; f.dzz[e] = 0.0;
+  f.delv_xi[e] = 0.0; f.delv_eta[e] = 0.0; f.delv_zeta[e] = 0.0;
+  f.delx_xi[e] = 0.0; f.delx_eta[e] = 0.0; f.delx_zeta[e] = 0.0;
+  // setupBoundaryConditions for the single rank: SYMM on min faces, FREE on max
+  const int64_t col = e % s, rest = e / s;
+  const int64_t row = rest % s, plane = rest / s;
+  uint32_t bc = 0;
+  if (plane == 0) bc |= ZETA_M_SYMM;
+  if (plane == s - 1) bc |= ZETA_P_FREE;
+  if (row == 0) bc |= ETA_M_SYMM;
+  if (row == s - 1) bc |= ETA_P_FREE;
+  if (col == 0) bc |= XI_M_SYMM;
+  if (col == s - 1) bc |= XI_P_FREE;
+  f.elemBC[e] = bc;
+}
+
+// nodal mass: volume/8 of each adjacent element added in ascending element
+// order, the order of the element loop at src/domain.jl:709-727
+__global__ void k_init_mass(int s, int64_t N, DevFields f) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const int64_t s1 = s + 1;
+  const int i = (int)(n % s1), j = (int)((n / s1) % s1), k = (int)(n / (s1 * s1));
+  double m = 0.0;
+  for (int dk = -1; dk <= 0; ++dk) {
+    const int ek = k + dk;
+    if (ek < 0 || ek >= s) continue;
+    for (int dj = -1; dj <= 0; ++dj) {
+      const int ej = j + dj;
+      if (ej < 0 || ej >= s) continue;
+      for (int di = -1; di <= 0; ++di) {
+        const int ei = i + di;
+        if (ei < 0 || ei >= s) continue;
+        const int64_t e = ((int64_t)ek * s + ej) * s + ei;
+        m = m + f.volo[e] / 8.0;
+      }
+    }
+  }
+  f.nodalMass[n] = m;
+}
+
+// scalars of src/domain.jl:663-701, energy deposit and dt0 (738-749)
+__global__ void k_init_scalars(int s, DevScalars *sc, double *e0, const double *volo) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  sc->dthydro = 1e20;
+  sc->dtcourant = 1e20;
+  sc->time = 0.;
+  sc->dtfixed = -1.0e-6;
+  sc->deltatimemultlb = 1.1;
+  sc->deltatimemultub = 1.2;
+  sc->stoptime = 1.0e-2;
+  sc->dtmax = 1.0e-2;
+  sc->cycle = 0;
+  sc->e_cut = 1.0e-7;
+  sc->p_cut = 1.0e-7;
+  sc->q_cut = 1.0e-7;
+  sc->u_cut = 1.0e-7;
+  sc->v_cut = 1.0e-10;
+  sc->hgcoef = 3.0;
+  sc->ss4o3 = 4.0 / 3.0;
+  sc->qstop = 1.0e+12;
+  sc->monoq_max_slope = 1.0;
+  sc->monoq_limiter_mult = 2.0;
+  sc->qlc_monoq = 0.5;
+  sc->qqc_monoq = 2.0 / 3.0;
+  sc->qqc = 2.0;
+  sc->pmin = 0.;
+  sc->emin = -1.0e+15;
+  sc->dvovmax = 0.1;
+  sc->eosvmax = 1.0e+9;
+  sc->eosvmin = 1.0e-9;
+  sc->refdens = 1.0;
+  const double ebase = 3.948746e+7;
+  const double scale = (double)s / 45.0;
+  const double einit = ((ebase * scale) * scale) * scale;
+  e0[0] = einit;
+  sc->deltatime = (.5 * cbrt_cr(volo[0])) / sqrt(2.0 * einit);
+  sc->err = ERR_NONE;
+  sc->active = 0;
+  sc->num_iters = 0;
+}
+
+// ---------------------------------------------------------------------------
+// the cycle
+// ---------------------------------------------------------------------------
+__global__ void k_time_increment(DevScalars *sc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) time_increment(sc);
+}
+
+// driver loop head (examples/benchmark.jl:82-84): the first iteration is
+// entered whenever time < stoptime
+__global__ void k_run_begin(DevScalars *sc, int64_t num_iters) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  sc->num_iters = num_iters;
+  sc->err = ERR_NONE;
+  const int go = sc->time < sc->stoptime;
+  sc->active = go;
+  if (go) time_increment(sc);
+}
+
+// corner forces of one element.  cf layout: [component][corner][element],
+// components 0..2 stress x,y,z and 3..5 hourglass x,y,z.
+__global__ __launch_bounds__(BLOCK) void k_force(DevScalars *sc, int gated, int s, int64_t E,
+                                                 DevFields f, double *__restrict__ cf, int quirks) {
+  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= E) return;
+  if (gated && !sc->active) return;
+  int64_t nid[8];
+  elem_nodes(e, s, nid);
+  double X[8], Y[8], Z[8];
+  gather8(f.x, nid, X); gather8(f.y, nid, Y); gather8(f.z, nid, Z);
+
+  // integrateStressForElems, src/domain.jl:1097-1108 (sig from 963-969)
+  const double detJ = jacobian_volume(X, Y, Z);
+  if (detJ <= 0.0) raise(sc, ERR_EARLY_VOLUME);
+  {
+    double B[8][3];
+    node_normals(X, Y, Z, quirks, B);
+    const double ms = -((-f.p[e]) - f.q[e]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int n = 0; n < 8; ++n) cf[(int64_t)(c * 8 + n) * E + e] = ms * B[n][c];
+  }
+
+  // calcHourglassControlForElems, src/domain.jl:1707-1729
+  const double v = f.v[e];
+  const double determ = f.volo[e] * v;
+  if (v <= 0.0) raise(sc, ERR_HG_VOLUME);
+  const double hourg = sc->hgcoef;
+  if (!(hourg > 0.0)) return;
+
+  // calcFBHourglassForceForElems, src/domain.jl:1505-1639
+  double hourgam[8][4];
+  {
+    double dvdx[8], dvdy[8], dvdz[8];
+    volume_derivative(X, Y, Z, quirks, dvdx, dvdy, dvdz);
+    const double volinv = 1.0 / determ;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      double hx = X[0] * gam(0, m), hy = Y[0] * gam(0, m), hz = Z[0] * gam(0, m);
+#pragma unroll
+      for (int n = 1; n < 8; ++n) {
+        hx = hx + X[n] * gam(n, m);
+        hy = hy + Y[n] * gam(n, m);
+        hz = hz + Z[n] * gam(n, m);
+      }
+#pragma unroll
+      for (int n = 0; n < 8; ++n)
+        hourgam[n][m] = gam(n, m) - volinv * (((dvdx[n] * hx) + (dvdy[n] * hy)) + (dvdz[n] * hz));
+    }
+  }
+  const double volume13 = cbrt_cr(determ);
+  const double coefficient = ((((-hourg) * 0.01) * f.ss[e]) * f.elemMass[e]) / volume13;
+  const double *vel[3] = {f.xd, f.yd, f.zd};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double V[8];
+    gather8(vel[c], nid, V);
+    double h[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      double acc = hourgam[0][m] * V[0];
+#pragma unroll
+      for (int n = 1; n < 8; ++n) acc = acc + hourgam[n][m] * V[n];
+      h[m] = acc;
+    }
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+      cf[(int64_t)((3 + c) * 8 + n) * E + e] =
+          coefficient * ((((hourgam[n][0] * h[0]) + (hourgam[n][1] * h[1])) + (hourgam[n][2] * h[2])) +
+                         (hourgam[n][3] * h[3]));
+  }
+}
+
+// corner of element (ei,ej,ek) that node (i,j,k) occupies, by node offset
+// (i-ei) + 2(j-ej) + 4(k-ek): the hex corner order of src/domain.jl:158-165
+LDEV int corner_of(int ox, int oy, int oz) {
+  // offsets (0,0,0)->0 (1,0,0)->1 (1,1,0)->2 (0,1,0)->3, +4 for oz
+  return 4 * oz + (oy ? (ox ? 2 : 3) : (ox ? 1 : 0));
+}
+
+// gathers of src/domain.jl:1112-1127 and 1678-1693, then
+// calcAccelerationForNodes .. calcPositionForNodes (1781-1838)
+__global__ __launch_bounds__(BLOCK) void k_node(DevScalars *sc, int gated, int s, int64_t N,
+                                                int64_t E, DevFields f, const double *__restrict__ cf,
+                                                int keep_aux) {
+  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (n >= N) return;
+  if (gated && !sc->active) return;
+  const int64_t s1 = s + 1;
+  const int i = (int)(n % s1), j = (int)((n / s1) % s1), k = (int)(n / (s1 * s1));
+  double fs0 = 0.0, fs1 = 0.0, fs2 = 0.0, fh0 = 0.0, fh1 = 0.0, fh2 = 0.0;
+  for (int dk = -1; dk <= 0; ++dk) {
+    const int ek = k + dk;
+    if (ek < 0 || ek >= s) continue;
+    for (int dj = -1; dj <= 0; ++dj) {
+      const int ej = j + dj;
+      if (ej < 0 || ej >= s) continue;
+      for (int di = -1; di <= 0; ++di) {
+        const int ei = i + di;
+        if (ei < 0 || ei >= s) continue;
+        const int64_t e = ((int64_t)ek * s + ej) * s + ei;
+        const int c = corner_of(-di, -dj, -dk);
+        fs0 = fs0 + cf[(int64_t)(0 * 8 + c) * E + e];
+        fs1 = fs1 + cf[(int64_t)(1 * 8 + c) * E + e];
+        fs2 = fs2 + cf[(int64_t)(2 * 8 + c) * E + e];
+        fh0 = fh0 + cf[(int64_t)(3 * 8 + c) * E + e];
+        fh1 = fh1 + cf[(int64_t)(4 * 8 + c) * E + e];
+        fh2 = fh2 + cf[(int64_t)(5 * 8 + c) * E + e];
+      }
+    }
+  }
+  double fx = fs0, fy = fs1, fz = fs2;
+  if (sc->hgcoef > 0.0) {
+    fx = fx + fh0;
+    fy = fy + fh1;
+    fz = fz + fh2;
+  }
+  const double mass = f.nodalMass[n];
+  double xdd = fx / mass, ydd = fy / mass, zdd = fz / mass;
+  // applyAccelerationBoundaryConditionsForNodes: symmX/Y/Z are the col 0,
+  // row 0 and plane 0 node sets (src/domain.jl:773-791)
+  if (i == 0) xdd = 0.0;
+  if (j == 0) ydd = 0.0;
+  if (k == 0) zdd = 0.0;
+  const double dt = sc->deltatime, u_cut = sc->u_cut;
+  double xd = f.xd[n] + xdd * dt;
+  if (fabs(xd) < u_cut) xd = 0.0;
+  double yd = f.yd[n] + ydd * dt;
+  if (fabs(yd) < u_cut) yd = 0.0;
+  double zd = f.zd[n] + zdd * dt;
+  if (fabs(zd) < u_cut) zd = 0.0;
+  f.xd[n] = xd;
+  f.yd[n] = yd;
+  f.zd[n] = zd;
+  f.x[n] = f.x[n] + xd * dt;
+  f.y[n] = f.y[n] + yd * dt;
+  f.z[n] = f.z[n] + zd * dt;
+  if (keep_aux) {
+    f.fx[n] = fx; f.fy[n] = fy; f.fz[n] = fz;
+    f.xdd[n] = xdd; f.ydd[n] = ydd; f.zdd[n] = zdd;
+  }
+}
+
+// calcKinematicsForElems (src/domain.jl:2099-2147), the deviatoric loop
+// (2156-2171) and calcMonotonicQGradientsForElems (2175-2320)
+__global__ __launch_bounds__(BLOCK) void k_kin(DevScalars *sc, int gated, int s, int64_t E,
+                                               DevFields f, int keep_aux) {
+  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= E) return;
+  if (gated && !sc->active) return;
+  int64_t nid[8];
+  elem_nodes(e, s, nid);
+  double X[8], Y[8], Z[8], XV[8], YV[8], ZV[8];
+  gather8(f.x, nid, X); gather8(f.y, nid, Y); gather8(f.z, nid, Z);
+  gather8(f.xd, nid, XV); gather8(f.yd, nid, YV); gather8(f.zd, nid, ZV);
+  const double volo = f.volo[e];
+  const double volume = elem_volume(X, Y, Z);
+  const double vnew = volume / volo;
+  if (vnew <= 0.0) raise(sc, ERR_KIN_VOLUME);
+  f.vnew[e] = vnew;
+  f.delv[e] = vnew - f.v[e];
+  f.arealg[e] = char_length(X, Y, Z, volume);
+  {
+    const double dt2 = 0.5 * sc->deltatime;
+    double XS[8], YS[8], ZS[8], b[4][3];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      XS[n] = X[n] - dt2 * XV[n];
+      YS[n] = Y[n] - dt2 * YV[n];
+      ZS[n] = Z[n] - dt2 * ZV[n];
+    }
+    const double detJ = shape_derivatives4(XS, YS, ZS, b);
+    const double inv_detJ = 1.0 / detJ;
+    const double d1 = inv_detJ * vgrad_term(b, 0, XV);
+    const double d2 = inv_detJ * vgrad_term(b, 1, YV);
+    const double d3 = inv_detJ * vgrad_term(b, 2, ZV);
+    const double vdov = (d1 + d2) + d3;
+    f.vdov[e] = vdov;
+    if (keep_aux) {
+      const double vdovthird = vdov / 3.0;
+      f.dxx[e] = d1 - vdovthird;
+      f.dyy[e] = d2 - vdovthird;
+      f.dzz[e] = d3 - vdovthird;
+    }
+    if (vnew <= 0.0) raise(sc, ERR_LAG_VOLUME);
+  }
+  // MonoQ gradients
+  const double ptiny = 1.e-36;
+  const double vol = volo * vnew;
+  const double norm = 1.0 / (vol + ptiny);
+  const double dxj = -0.25 * (sum4(X[0], X[1], X[5], X[4]) - sum4(X[3], X[2], X[6], X[7]));
+  const double dyj = -0.25 * (sum4(Y[0], Y[1], Y[5], Y[4]) - sum4(Y[3], Y[2], Y[6], Y[7]));
+  const double dzj = -0.25 * (sum4(Z[0], Z[1], Z[5], Z[4]) - sum4(Z[3], Z[2], Z[6], Z[7]));
+  const double dxi = 0.25 * (sum4(X[1], X[2], X[6], X[5]) - sum4(X[0], X[3], X[7], X[4]));
+  const double dyi = 0.25 * (sum4(Y[1], Y[2], Y[6], Y[5]) - sum4(Y[0], Y[3], Y[7], Y[4]));
+  const double dzi = 0.25 * (sum4(Z[1], Z[2], Z[6], Z[5]) - sum4(Z[0], Z[3], Z[7], Z[4]));
+  const double dxk = 0.25 * (sum4(X[4], X[5], X[6], X[7]) - sum4(X[0], X[1], X[2], X[3]));
+  const double dyk = 0.25 * (sum4(Y[4], Y[5], Y[6], Y[7]) - sum4(Y[0], Y[1], Y[2], Y[3]));
+  const double dzk = 0.25 * (sum4(Z[4], Z[5], Z[6], Z[7]) - sum4(Z[0], Z[1], Z[2], Z[3]));
+  double ax, ay, az, dxv, dyv, dzv;
+  // zeta: i cross j
+  ax = (dyi * dzj) - (dzi * dyj);
+  ay = (dzi * dxj) - (dxi * dzj);
+  az = (dxi * dyj) - (dyi * dxj);
+  f.delx_zeta[e] = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  ax = ax * norm; ay = ay * norm; az = az * norm;
+  dxv = 0.25 * (sum4(XV[4], XV[5], XV[6], XV[7]) - sum4(XV[0], XV[1], XV[2], XV[3]));
+  dyv = 0.25 * (sum4(YV[4], YV[5], YV[6], YV[7]) - sum4(YV[0], YV[1], YV[2], YV[3]));
+  dzv = 0.25 * (sum4(ZV[4], ZV[5], ZV[6], ZV[7]) - sum4(ZV[0], ZV[1], ZV[2], ZV[3]));
+  f.delv_zeta[e] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+  // xi: j cross k
+  ax = (dyj * dzk) - (dzj * dyk);
+  ay = (dzj * dxk) - (dxj * dzk);
+  az = (dxj * dyk) - (dyj * dxk);
+  f.delx_xi[e] = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  ax = ax * norm; ay = ay * norm; az = az * norm;
+  dxv = 0.25 * (sum4(XV[1], XV[2], XV[6], XV[5]) - sum4(XV[0], XV[3], XV[7], XV[4]));
+  dyv = 0.25 * (sum4(YV[1], YV[2], YV[6], YV[5]) - sum4(YV[0], YV[3], YV[7], YV[4]));
+  dzv = 0.25 * (sum4(ZV[1], ZV[2], ZV[6], ZV[5]) - sum4(ZV[0], ZV[3], ZV[7], ZV[4]));
+  f.delv_xi[e] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+  // eta: k cross i
+  ax = (dyk * dzi) - (dzk * dyi);
+  ay = (dzk * dxi) - (dxk * dzi);
+  az = (dxk * dyi) - (dyk * dxi);
+  f.delx_eta[e] = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  ax = ax * norm; ay = ay * norm; az = az * norm;
+  dxv = -0.25 * (sum4(XV[0], XV[1], XV[5], XV[4]) - sum4(XV[3], XV[2], XV[6], XV[7]));
+  dyv = -0.25 * (sum4(YV[0], YV[1], YV[5], YV[4]) - sum4(YV[3], YV[2], YV[6], YV[7]));
+  dzv = -0.25 * (sum4(ZV[0], ZV[1], ZV[5], ZV[4]) - sum4(ZV[3], ZV[2], ZV[6], ZV[7]));
+  f.delv_eta[e] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+}
+
+// neighbour ids exactly as setupElementConnectivities stores them
+// (src/domain.jl:793-820), 0-based
+LDEV int64_t nb_xim(int64_t e) { return e == 0 ? 1 : e - 1; }
+LDEV int64_t nb_xip(int64_t e, int64_t E) { return e < E - 1 ? e + 1 : E - 1; }
+LDEV int64_t nb_etam(int64_t e, int64_t s) { return e < s ? e : e - s; }
+LDEV int64_t nb_etap(int64_t e, int64_t s, int64_t E) { return e < E - s ? e + s : e - (E - s); }
+LDEV int64_t nb_zetam(int64_t e, int64_t s2) { return e < s2 ? e : e - s2; }
+LDEV int64_t nb_zetap(int64_t e, int64_t s2, int64_t E) { return e < E - s2 ? e + s2 : e; }
+// a read the reference would make out of bounds (only reachable with an
+// uploaded elemBC that marks a boundary face interior, or s = 1) is clamped
+LDEV int64_t clampi(int64_t i, int64_t E) { return i < 0 ? 0 : (i >= E ? E - 1 : i); }
+
+// block-wide minimum of two values (64-wide wavefronts, BLOCK threads)
+LDEV void block_min2(double &a, double &b) {
+  __shared__ double sa[BLOCK / 64], sb[BLOCK / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double oa = __shfl_xor(a, off, 64);
+    const double ob = __shfl_xor(b, off, 64);
+    a = oa < a ? oa : a;
+    b = ob < b ? ob : b;
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) { sa[w] = a; sb[w] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < BLOCK / 64; ++i) {
+      a = sa[i] < a ? sa[i] : a;
+      b = sb[i] < b ? sb[i] : b;
+    }
+  }
+}
+
+// calcMonotonicQRegionForElems (src/domain.jl:2322-2513, quirk Q3), the
+// QStop check (2567-2582), applyMaterialPropertiesForElems + evalEOSForElems
+// + calcEnergyForElems + calcSoundSpeedForElems (2585-2910, one element at a
+// time; the reference's element-wise temporaries never mix elements),
+// updateVolumesForElems (2912-2930) and the per-element terms of
+// calcCourantConstraintForElems / calcHydroConstraintForElems (2964-3054)
+__global__ __launch_bounds__(BLOCK) void k_qeos(DevScalars *sc, int gated, int s, int64_t E,
+                                                DevFields f, double *__restrict__ dtpart, int quirks,
+                                                int keep_aux) {
+  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (gated && !sc->active) return;  // uniform over the grid
+  double cand_c = 1.0e+20, cand_h = 1.0e+20;
+  if (e < E) {
+    const double ptiny = 1e-36;
+    const double mult = sc->monoq_limiter_mult, maxslope = sc->monoq_max_slope;
+    const uint32_t bc = f.elemBC[e];
+    const double dvxi = f.delv_xi[e], dveta = f.delv_eta[e], dvzeta = f.delv_zeta[e];
+    bool bad = false;
+    double delvm = 0.0, delvp = 0.0;
+    uint32_t c;
+    // xi
+    c = bc & XI_M;
+    if (c == 0 || c == XI_M_COMM) delvm = f.delv_xi[clampi(nb_xim(e), E)];
+    else if (c == XI_M_SYMM) delvm = dvxi;
+    else if (c == XI_M_FREE) delvm = 0.0;
+    else { delvm = 0.0; bad = true; }
+    c = bc & XI_P;
+    if (c == 0 || c == XI_P_COMM) {
+      const int64_t nb = nb_xip(e, E);  // stored 0-based id
+      delvp = (quirks & Q3) ? f.delv_xi[clampi(nb - 1, E)] : f.delv_xi[clampi(nb, E)];
+    } else if (c == XI_P_SYMM) delvp = dvxi;
+    else if (c == XI_P_FREE) delvp = 0.0;
+    else { delvp = 0.0; bad = true; }
+    const double phixi = limit_phi(delvm, delvp, 1.0 / (dvxi + ptiny), mult, maxslope);
+    // eta
+    c = bc & ETA_M;
+    if (c == 0 || c == ETA_M_COMM) delvm = f.delv_eta[clampi(nb_etam(e, s), E)];
+    else if (c == ETA_M_SYMM) delvm = dveta;
+    else if (c == ETA_M_FREE) delvm = 0.0;
+    else { delvm = 0.0; bad = true; }
+    c = bc & ETA_P;
+    if (c == 0 || c == ETA_P_COMM) delvp = f.delv_eta[clampi(nb_etap(e, s, E), E)];
+    else if (c == ETA_P_SYMM) delvp = dveta;
+    else if (c == ETA_P_FREE) delvp = 0.0;
+    else { delvp = 0.0; bad = true; }
+    const double phieta = limit_phi(delvm, delvp, 1.0 / (dveta + ptiny), mult, maxslope);
+    // zeta
+    const int64_t s2 = (int64_t)s * s;
+    c = bc & ZETA_M;
+    if (c == 0 || c == ZETA_M_COMM) delvm = f.delv_zeta[clampi(nb_zetam(e, s2), E)];
+    else if (c == ZETA_M_SYMM) delvm = dvzeta;
+    else if (c == ZETA_M_FREE) delvm = 0.0;
+    else { delvm = 0.0; bad = true; }
+    c = bc & ZETA_P;
+    if (c == 0 || c == ZETA_P_COMM) delvp = f.delv_zeta[clampi(nb_zetap(e, s2, E), E)];
+    else if (c == ZETA_P_SYMM) delvp = dvzeta;
+    else if (c == ZETA_P_FREE) delvp = 0.0;
+    else { delvp = 0.0; bad = true; }
+    const double phizeta = limit_phi(delvm, delvp, 1.0 / (dvzeta + ptiny), mult, maxslope);
+    if (bad) raise(sc, ERR_BC);
+
+    const double vdov = f.vdov[e], volo = f.volo[e], vnew = f.vnew[e];
+    double qlin, qquad;
+    if (vdov > 0.0) {
+      qlin = 0.0;
+      qquad = 0.0;
+    } else {
+      double delvxxi = dvxi * f.delx_xi[e];
+      double delvxeta = dveta * f.delx_eta[e];
+      double delvxzeta = dvzeta * f.delx_zeta[e];
+      if (delvxxi > 0.0) delvxxi = 0.0;
+      if (delvxeta > 0.0) delvxeta = 0.0;
+      if (delvxzeta > 0.0) delvxzeta = 0.0;
+      const double rho = f.elemMass[e] / (volo * vnew);
+      qlin = ((-sc->qlc_monoq) * rho) *
+             (((delvxxi * (1.0 - phixi)) + (delvxeta * (1.0 - phieta))) + (delvxzeta * (1.0 - phizeta)));
+      qquad = (sc->qqc_monoq * rho) *
+              ((((delvxxi * delvxxi) * (1.0 - (phixi * phixi))) +
+                ((delvxeta * delvxeta) * (1.0 - (phieta * phieta)))) +
+               ((delvxzeta * delvxzeta) * (1.0 - (phizeta * phizeta))));
+    }
+    if (keep_aux) {
+      f.qq[e] = qquad;
+      f.ql[e] = qlin;
+    }
+
+    // QStop on the q of the previous cycle
+    double q_old = f.q[e];
+    if (q_old > sc->qstop) raise(sc, ERR_QSTOP);
+
+    // material properties
+    const double eosvmin = sc->eosvmin, eosvmax = sc->eosvmax;
+    double vc = f.v[e];
+    if (eosvmin != 0.0 && vc < eosvmin) vc = eosvmin;
+    if (eosvmax != 0.0 && vc > eosvmax) vc = eosvmax;
+    if (vc <= 0.0) raise(sc, ERR_EOS_VOLUME);
+    double vnewc = vnew;
+    if (eosvmin != 0.0 && vnewc < eosvmin) vnewc = eosvmin;
+    if (eosvmax != 0.0 && vnewc > eosvmax) vnewc = eosvmax;
+
+    const double delvc = f.delv[e], e_old = f.e[e];
+    double p_old = f.p[e];
+    const double TINY1 = 0.111111e-36, TINY3 = 0.333333e-18, SIXTH = 1.0 / 6.0;
+    const double pmin = sc->pmin, p_cut = sc->p_cut, e_cut = sc->e_cut, q_cut = sc->q_cut;
+    const double emin = sc->emin, rho0 = sc->refdens;
+    double compression = 1.0 / vnewc - 1.0;
+    const double vchalf = vnewc - delvc * 0.5;
+    double compHalfStep = (1.0 / vchalf) - 1.0;
+    if (eosvmin != 0.0 && vnewc <= eosvmin) compHalfStep = compression;
+    if (eosvmax != 0.0 && vnewc >= eosvmax) {
+      p_old = 0.0;
+      compression = 0.0;
+      compHalfStep = 0.0;
+    }
+    const double work = 0.0;
+    double bvc, pbvc;
+    double e_new = (e_old - ((0.5 * delvc) * (p_old + q_old))) + (0.5 * work);
+    if (e_new < emin) e_new = emin;
+    const double pHalfStep = pressure(bvc, pbvc, e_new, compHalfStep, vnewc, pmin, p_cut, eosvmax);
+    const double vhalf = 1.0 / (1.0 + compHalfStep);
+    double q_new;
+    if (delvc > 0.0) {
+      q_new = 0.0;
+    } else {
+      double ssc = ((pbvc * e_new) + (((vhalf * vhalf) * bvc) * pHalfStep)) / rho0;
+      ssc = (ssc <= TINY1) ? TINY3 : sqrt(ssc);
+      q_new = (ssc * qlin) + qquad;
+    }
+    e_new = e_new + ((0.5 * delvc) * ((3.0 * (p_old + q_old)) - (4.0 * (pHalfStep + q_new))));
+    e_new = e_new + 0.5 * work;
+    if (fabs(e_new) < e_cut) e_new = 0.0;
+    if (e_new < emin) e_new = emin;
+    double p_new = pressure(bvc, pbvc, e_new, compression, vnewc, pmin, p_cut, eosvmax);
+    double q_tilde;
+    if (delvc > 0.0) {
+      q_tilde = 0.0;
+    } else {
+      double ssc = ((pbvc * e_new) + (((vnewc * vnewc) * bvc) * p_new)) / rho0;
+      ssc = (ssc <= TINY1) ? TINY3 : sqrt(ssc);
+      q_tilde = (ssc * qlin) + qquad;
+    }
+    e_new = e_new - (((((7.0 * (p_old + q_old)) - (8.0 * (pHalfStep + q_new))) + (p_new + q_tilde)) *
+                      delvc) *
+                     SIXTH);
+    if (fabs(e_new) < e_cut) e_new = 0.0;
+    if (e_new < emin) e_new = emin;
+    p_new = pressure(bvc, pbvc, e_new, compression, vnewc, pmin, p_cut, eosvmax);
+    if (delvc <= 0.0) {
+      double ssc = ((pbvc * e_new) + (((vnewc * vnewc) * bvc) * p_new)) / rho0;
+      ssc = (ssc <= TINY1) ? TINY3 : sqrt(ssc);
+      q_new = (ssc * qlin) + qquad;
+      if (fabs(q_new) < q_cut) q_new = 0.0;
+    }
+    double ss = ((pbvc * e_new) + (((vnewc * vnewc) * bvc) * p_new)) / rho0;
+    ss = (ss <= TINY1) ? TINY3 : sqrt(ss);
+    f.p[e] = p_new;
+    f.e[e] = e_new;
+    f.q[e] = q_new;
+    f.ss[e] = ss;
+
+    // updateVolumesForElems
+    double tmpV = vnew;
+    if (fabs(tmpV - 1.0) < sc->v_cut) tmpV = 1.0;
+    f.v[e] = tmpV;
+    if (tmpV <= 0.0) raise(sc, ERR_UPDATE_VOLUME);
+
+    // time-constraint terms; an element only counts where vdov != 0 and its
+    // value beats the 1e20 start value (strict <, as in the reference)
+    if (vdov != 0.0) {
+      const double qqc = sc->qqc;
+      const double qqc2 = (64.0 * qqc) * qqc;
+      const double arealg = f.arealg[e];
+      double dtf = ss * ss;
+      if (vdov < 0.0) dtf = dtf + ((((qqc2 * arealg) * arealg) * vdov) * vdov);
+      dtf = sqrt(dtf);
+      dtf = arealg / dtf;
+      if (dtf < cand_c) cand_c = dtf;
+      const double dtdvov = sc->dvovmax / (fabs(vdov) + 1.e-20);
+      if (dtdvov < cand_h) cand_h = dtdvov;
+    }
+  }
+  block_min2(cand_c, cand_h);
+  if (threadIdx.x == 0) {
+    dtpart[2 * blockIdx.x] = cand_c;
+    dtpart[2 * blockIdx.x + 1] = cand_h;
+  }
+}
+
+// per-element time-constraint terms from the fields alone, for a
+// calcTimeConstraintsForElems call that does not follow k_qeos
+__global__ __launch_bounds__(BLOCK) void k_dtpart(DevScalars *sc, int64_t E, DevFields f,
+                                                  double *__restrict__ dtpart) {
+  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  double cand_c = 1.0e+20, cand_h = 1.0e+20;
+  if (e < E) {
+    const double vdov = f.vdov[e];
+    if (vdov != 0.0) {
+      const double qqc2 = (64.0 * sc->qqc) * sc->qqc;
+      const double arealg = f.arealg[e], ss = f.ss[e];
+      double dtf = ss * ss;
+      if (vdov < 0.0) dtf = dtf + ((((qqc2 * arealg) * arealg) * vdov) * vdov);
+      dtf = sqrt(dtf);
+      dtf = arealg / dtf;
+      if (dtf < cand_c) cand_c = dtf;
+      const double dtdvov = sc->dvovmax / (fabs(vdov) + 1.e-20);
+      if (dtdvov < cand_h) cand_h = dtdvov;
+    }
+  }
+  block_min2(cand_c, cand_h);
+  if (threadIdx.x == 0) {
+    dtpart[2 * blockIdx.x] = cand_c;
+    dtpart[2 * blockIdx.x + 1] = cand_h;
+  }
+}
+
+// final minima; in driver-loop mode also the loop test of
+// examples/benchmark.jl:82,95 and the timeIncrement! of the next cycle
+__global__ __launch_bounds__(1024) void k_dt(DevScalars *sc, int gated, int64_t nblocks,
+                                             const double *__restrict__ dtpart, int loop_mode) {
+  if (gated && !sc->active) return;
+  double a = 1.0e+20, b = 1.0e+20;
+  for (int64_t i = threadIdx.x; i < nblocks; i += blockDim.x) {
+    const double pa = dtpart[2 * i], pb = dtpart[2 * i + 1];
+    a = pa < a ? pa : a;
+    b = pb < b ? pb : b;
+  }
+  __shared__ double sa[16], sb[16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double oa = __shfl_xor(a, off, 64);
+    const double ob = __shfl_xor(b, off, 64);
+    a = oa < a ? oa : a;
+    b = ob < b ? ob : b;
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) { sa[w] = a; sb[w] = b; }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+    a = sa[i] < a ? sa[i] : a;
+    b = sb[i] < b ? sb[i] : b;
+  }
+  if (a < 1.0e+20) sc->dtcourant = a;
+  if (b < 1.0e+20) sc->dthydro = b;
+  if (loop_mode) {
+    const int go = (sc->err == ERR_NONE) && (sc->cycle < sc->num_iters) && (sc->time < sc->stoptime);
+    sc->active = go;
+    if (go) time_increment(sc);
+  }
+}
+
+__global__ void k_math(int op, const double *__restrict__ a, const double *__restrict__ b,
+                       double *__restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = a[i];
+  out[i] = op == 0 ? cbrt_cr(x) : (op == 1 ? sqrt(x) : x / b[i]);
+}
+
+__global__ void k_clear_err(DevScalars *sc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) sc->err = ERR_NONE;
+}
+
+}  // namespace lulesh
+
+// ---------------------------------------------------------------------------
+// launchers (called from the C ABI)
+// ---------------------------------------------------------------------------
+namespace lulesh {
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_init(hipStream_t st, int s, int64_t E, int64_t N, const DevFields &f, DevScalars *sc) {
+  hipLaunchKernelGGL(k_init_nodes, dim3(nblk(N, BLOCK)), dim3(BLOCK), 0, st, s, N, f);
+  hipLaunchKernelGGL(k_init_elems, dim3(nblk(E, BLOCK)), dim3(BLOCK), 0, st, s, E, f);
+  hipLaunchKernelGGL(k_init_mass, dim3(nblk(N, BLOCK)), dim3(BLOCK), 0, st, s, N, f);
+  hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, s, sc, f.e, f.volo);
+  return hipGetLastError();
+}
+
+int64_t qeos_blocks(int64_t E) { return nblk(E, BLOCK); }
+
+hipError_t launch_math(hipStream_t st, int op, const double *a, const double *b, double *out,
+                       int64_t n) {
+  hipLaunchKernelGGL(k_math, dim3(nblk(n, BLOCK)), dim3(BLOCK), 0, st, op, a, b, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_kernel(int id, hipStream_t st, const LaunchArgs &a) {
+  switch (id) {
+    case KID_FORCE:
+      hipLaunchKernelGGL(k_force, dim3(nblk(a.E, BLOCK)), dim3(BLOCK), 0, st, a.sc, a.gated, a.s,
+                         a.E, a.f, a.cf, a.quirks);
+      break;
+    case KID_NODE:
+      hipLaunchKernelGGL(k_node, dim3(nblk(a.N, BLOCK)), dim3(BLOCK), 0, st, a.sc, a.gated, a.s,
+                         a.N, a.E, a.f, a.cf, a.keep_aux);
+      break;
+    case KID_KIN:
+      hipLaunchKernelGGL(k_kin, dim3(nblk(a.E, BLOCK)), dim3(BLOCK), 0, st, a.sc, a.gated, a.s,
+                         a.E, a.f, a.keep_aux);
+      break;
+    case KID_QEOS:
+      hipLaunchKernelGGL(k_qeos, dim3(nblk(a.E, BLOCK)), dim3(BLOCK), 0, st, a.sc, a.gated, a.s,
+                         a.E, a.f, a.dtpart, a.quirks, a.keep_aux);
+      break;
+    case KID_DT:
+      hipLaunchKernelGGL(k_dt, dim3(1), dim3(1024), 0, st, a.sc, a.gated, qeos_blocks(a.E),
+                         a.dtpart, a.loop_mode);
+      break;
+    case KID_DTPART:
+      hipLaunchKernelGGL(k_dtpart, dim3(nblk(a.E, BLOCK)), dim3(BLOCK), 0, st, a.sc, a.E, a.f,
+                         a.dtpart);
+      break;
+    case KID_TIME_INCREMENT:
+      hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, st, a.sc);
+      break;
+    case KID_RUN_BEGIN:
+      hipLaunchKernelGGL(k_run_begin, dim3(1), dim3(64), 0, st, a.sc, a.num_iters);
+      break;
+    case KID_CLEAR_ERR:
+      hipLaunchKernelGGL(k_clear_err, dim3(1), dim3(64), 0, st, a.sc);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace lulesh

@@ -1,0 +1,66 @@
+"""ctypes binding of liblulesh_hip.so (the C ABI in include/lulesh_hip.h).
+
+The library is the product: there is no CPU fallback.  If it has not been
+built, importing this module raises immediately.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "liblulesh_hip.so"))
+
+c_int, c_int32, c_int64, c_double = ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+c_char_p, c_void_p = ctypes.c_char_p, ctypes.c_void_p
+P_double = ctypes.POINTER(c_double)
+P_int64 = ctypes.POINTER(c_int64)
+P_uint32 = ctypes.POINTER(ctypes.c_uint32)
+
+# name -> (restype, argtypes); mirrors include/lulesh_hip.h
+SIGNATURES = {
+    "lulesh_create": (c_int, [c_int32, c_int32, c_int32, c_int32, c_int32, c_int32,
+                              ctypes.POINTER(c_void_p)]),
+    "lulesh_destroy": (c_int, [c_void_p]),
+    "lulesh_reset": (c_int, [c_void_p]),
+    "lulesh_num_elem": (c_int64, [c_void_p]),
+    "lulesh_num_node": (c_int64, [c_void_p]),
+    "lulesh_last_error": (c_char_p, [c_void_p]),
+    "lulesh_error_string": (c_char_p, [c_int]),
+    "lulesh_download": (c_int, [c_void_p, c_char_p, P_double, c_int64]),
+    "lulesh_upload": (c_int, [c_void_p, c_char_p, P_double, c_int64]),
+    "lulesh_download_elem_bc": (c_int, [c_void_p, P_uint32, c_int64]),
+    "lulesh_upload_elem_bc": (c_int, [c_void_p, P_uint32, c_int64]),
+    "lulesh_get_scalar": (c_int, [c_void_p, c_char_p, P_double]),
+    "lulesh_set_scalar": (c_int, [c_void_p, c_char_p, c_double]),
+    "lulesh_set_keep_aux": (c_int, [c_void_p, c_int32]),
+    "lulesh_time_increment": (c_int, [c_void_p]),
+    "lulesh_lagrange_leapfrog": (c_int, [c_void_p]),
+    "lulesh_lagrange_nodal": (c_int, [c_void_p]),
+    "lulesh_lagrange_elements": (c_int, [c_void_p]),
+    "lulesh_calc_time_constraints": (c_int, [c_void_p]),
+    "lulesh_run": (c_int, [c_void_p, c_int64, P_int64]),
+    "lulesh_verify": (c_int, [c_void_p, P_double]),
+    "lulesh_set_timing": (c_int, [c_void_p, c_int32]),
+    "lulesh_kernel_time": (c_int, [c_void_p, c_int32, P_double, P_int64]),
+    "lulesh_kernel_name": (c_char_p, [c_int32]),
+    "lulesh_math_check": (c_int, [c_int32, P_double, P_double, P_double, c_int64, c_int32]),
+}
+
+_lib = None
+
+
+def load():
+    """Load and type the library once; raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"liblulesh_hip.so not found at {LIB_PATH}; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` or `make -C lulesh.jl_amd/csrc`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

@@ -1,0 +1,270 @@
+"""Host-side mirror of LULESH.jl's Domain and driver API, backed by the
+MI355X kernels of liblulesh_hip.
+
+Names, argument meaning and error behaviour follow the reference:
+  LuleshProblem        src/LULESH.jl:31-49
+  Domain(prob)         src/domain.jl:467-771   (built on the device)
+  timeIncrement!       src/domain.jl:920-961   -> timeIncrement(domain)
+  lagrangeLeapFrog     src/domain.jl:3067-3080
+  verifyAndWriteFinalOutput  src/utils.jl:124-164
+  main (driver loop)   examples/benchmark.jl:22-111
+A physics error raises LuleshError carrying the reference's error() text.
+Fields are read and written by their reference names (domain.e,
+domain.x, domain.time ...); each access copies between host and device.
+"""
+import ctypes
+import time as _time
+
+import numpy as np
+
+from . import _lib
+
+QUIRK_FACE5_NORMAL = 1
+QUIRK_DVDX_SWAP = 2
+QUIRK_XIP_SELF = 4
+QUIRKS_REFERENCE = 7
+
+ELEM_FIELDS = ("e", "p", "q", "ql", "qq", "v", "volo", "delv", "vdov", "arealg", "ss",
+               "elemMass", "vnew", "delv_xi", "delv_eta", "delv_zeta", "delx_xi", "delx_eta",
+               "delx_zeta", "dxx", "dyy", "dzz")
+NODE_FIELDS = ("x", "y", "z", "xd", "yd", "zd", "xdd", "ydd", "zdd", "fx", "fy", "fz",
+               "nodalMass")
+SCALARS = ("time", "deltatime", "cycle", "dtcourant", "dthydro", "stoptime", "dtmax",
+           "dtfixed", "deltatimemultlb", "deltatimemultub", "u_cut", "hgcoef", "qstop",
+           "monoq_max_slope", "monoq_limiter_mult", "e_cut", "p_cut", "ss4o3", "q_cut",
+           "v_cut", "qlc_monoq", "qqc_monoq", "qqc", "eosvmax", "eosvmin", "pmin", "emin",
+           "dvovmax", "refdens")
+
+KERNELS = ("k_force", "k_node", "k_kin", "k_qeos", "k_dt")
+
+
+class LuleshError(Exception):
+    """Raised where the reference calls error(msg); str(exc) is msg."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class LuleshProblem:
+    """LuleshProblem(num_iters, structured, nx, nr, balance, cost, devicetype,
+    floattype, comm) of src/LULESH.jl:31-49, for one rank (comm=None)."""
+
+    def __init__(self, num_iters, structured, nx, nr=11, balance=1, cost=1, devicetype="hip",
+                 floattype=np.float64, comm=None):
+        if comm is not None:
+            raise NotImplementedError("one single-GPU domain: comm must be None")
+        if floattype not in (np.float64, float):
+            raise ValueError("the path is FP64 only")
+        self.num_iters = int(num_iters)
+        self.structured = bool(structured)
+        self.nx = int(nx)
+        self.nr, self.balance, self.cost = int(nr), int(balance), int(cost)
+        self.devicetype, self.floattype, self.comm = devicetype, np.float64, None
+        # initMeshDecomp for one rank (src/LULESH.jl:62-97)
+        self.col = self.row = self.plane = 0
+        self.side = 1
+
+
+def _check(lib, ctx, rc):
+    if rc != 0:
+        msg = lib.lulesh_last_error(ctx) if ctx else lib.lulesh_error_string(rc)
+        msg = msg.decode() if isinstance(msg, bytes) else str(msg)
+        raise LuleshError(rc, msg)
+
+
+class Domain:
+    """GPU-resident Domain{Float64}; Domain(prob) of src/domain.jl:467-771."""
+
+    def __init__(self, prob, quirks=QUIRKS_REFERENCE, device=0):
+        if not prob.structured:
+            # src/domain.jl:611-613
+            raise LuleshError(0, "Reading unstructured mesh is currently missing in the Julia "
+                                 "version of LULESH.")
+        lib = _lib.load()
+        self._lib = lib
+        ctx = ctypes.c_void_p()
+        rc = lib.lulesh_create(prob.nx, prob.nr, prob.balance, prob.cost, int(quirks),
+                               int(device), ctypes.byref(ctx))
+        if rc != 0:
+            raise LuleshError(rc, lib.lulesh_error_string(rc).decode())
+        self._ctx = ctx
+        self.prob = prob
+        self.quirks = int(quirks)
+        self.sizeX = self.sizeY = self.sizeZ = prob.nx
+        self.numElem = int(lib.lulesh_num_elem(ctx))
+        self.numNode = int(lib.lulesh_num_node(ctx))
+        self.numReg, self.balance, self.cost = prob.nr, prob.balance, prob.cost
+        self.comm = None
+
+    # -- lifetime --
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.lulesh_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        _check(self._lib, self._ctx, self._lib.lulesh_reset(self._ctx))
+
+    # -- field access by reference name --
+    def _len(self, name):
+        if name in ELEM_FIELDS:
+            return self.numElem
+        if name in NODE_FIELDS:
+            return self.numNode
+        raise AttributeError(name)
+
+    def get(self, name):
+        if name in SCALARS:
+            v = ctypes.c_double()
+            _check(self._lib, self._ctx,
+                   self._lib.lulesh_get_scalar(self._ctx, name.encode(), ctypes.byref(v)))
+            return int(v.value) if name == "cycle" else v.value
+        if name == "elemBC":
+            out = np.empty(self.numElem, dtype=np.uint32)
+            _check(self._lib, self._ctx, self._lib.lulesh_download_elem_bc(
+                self._ctx, out.ctypes.data_as(_lib.P_uint32), out.size))
+            return out
+        n = self._len(name)
+        out = np.empty(n, dtype=np.float64)
+        _check(self._lib, self._ctx, self._lib.lulesh_download(
+            self._ctx, name.encode(), out.ctypes.data_as(_lib.P_double), n))
+        return out
+
+    def set(self, name, value):
+        if name in SCALARS:
+            _check(self._lib, self._ctx,
+                   self._lib.lulesh_set_scalar(self._ctx, name.encode(), float(value)))
+            return
+        if name == "elemBC":
+            a = np.ascontiguousarray(value, dtype=np.uint32)
+            _check(self._lib, self._ctx, self._lib.lulesh_upload_elem_bc(
+                self._ctx, a.ctypes.data_as(_lib.P_uint32), a.size))
+            return
+        a = np.ascontiguousarray(value, dtype=np.float64)
+        _check(self._lib, self._ctx, self._lib.lulesh_upload(
+            self._ctx, name.encode(), a.ctypes.data_as(_lib.P_double), a.size))
+
+    def __getattr__(self, name):
+        if name.startswith("_") or name in ("prob",):
+            raise AttributeError(name)
+        if name in SCALARS or name in ELEM_FIELDS or name in NODE_FIELDS or name == "elemBC":
+            return self.get(name)
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if name in SCALARS or name in ELEM_FIELDS or name in NODE_FIELDS or name == "elemBC":
+            self.set(name, value)
+        else:
+            object.__setattr__(self, name, value)
+
+    def keep_aux(self, on=True):
+        """Also write fx, xdd, dxx, ql, qq ... (off by default for speed)."""
+        _check(self._lib, self._ctx, self._lib.lulesh_set_keep_aux(self._ctx, 1 if on else 0))
+
+    # -- measurement --
+    def set_timing(self, on=True):
+        _check(self._lib, self._ctx, self._lib.lulesh_set_timing(self._ctx, 1 if on else 0))
+
+    def kernel_times(self):
+        out = {}
+        for k, name in enumerate(KERNELS):
+            ms, cnt = ctypes.c_double(), ctypes.c_int64()
+            _check(self._lib, self._ctx, self._lib.lulesh_kernel_time(
+                self._ctx, k, ctypes.byref(ms), ctypes.byref(cnt)))
+            out[name] = (ms.value, cnt.value)
+        return out
+
+
+# -- the hot path, by the reference's names --
+def timeIncrement(domain):
+    """timeIncrement!(domain), src/domain.jl:920-961."""
+    _check(domain._lib, domain._ctx, domain._lib.lulesh_time_increment(domain._ctx))
+
+
+def lagrangeLeapFrog(domain):
+    """lagrangeLeapFrog(domain), src/domain.jl:3067-3080."""
+    _check(domain._lib, domain._ctx, domain._lib.lulesh_lagrange_leapfrog(domain._ctx))
+
+
+def lagrangeNodal(domain):
+    """lagrangeNodal(domain), src/domain.jl:1840-1872."""
+    _check(domain._lib, domain._ctx, domain._lib.lulesh_lagrange_nodal(domain._ctx))
+
+
+def lagrangeElements(domain):
+    """lagrangeElements(domain), src/domain.jl:2932-2962."""
+    _check(domain._lib, domain._ctx, domain._lib.lulesh_lagrange_elements(domain._ctx))
+
+
+def calcTimeConstraintsForElems(domain):
+    """calcTimeConstraintsForElems(domain), src/domain.jl:3058-3064."""
+    _check(domain._lib, domain._ctx, domain._lib.lulesh_calc_time_constraints(domain._ctx))
+
+
+def run(domain, num_iters):
+    """The loop of examples/benchmark.jl:82-98 on the device; returns cycles run."""
+    n = ctypes.c_int64()
+    _check(domain._lib, domain._ctx,
+           domain._lib.lulesh_run(domain._ctx, int(num_iters), ctypes.byref(n)))
+    return n.value
+
+
+def verify(domain):
+    """(e[1], MaxAbsDiff, TotalAbsDiff, MaxRelDiff) of src/utils.jl:131-152."""
+    out = (ctypes.c_double * 4)()
+    _check(domain._lib, domain._ctx, domain._lib.lulesh_verify(domain._ctx, out))
+    return tuple(out)
+
+
+def verifyAndWriteFinalOutput(elapsed_time, domain, nx, numRanks=1, file=None):
+    """Prints the reference's final report (src/utils.jl:124-164)."""
+    import sys
+    f = file or sys.stdout
+    cycle = domain.cycle
+    e1, maxabs, totabs, maxrel = verify(domain)
+    grind1 = ((elapsed_time * 1e6) / cycle) / (nx * nx * nx) if cycle else float("nan")
+    grind2 = ((elapsed_time * 1e6) / cycle) / (nx * nx * nx * numRanks) if cycle else float("nan")
+    print("Run completed:", file=f)
+    print("   Problem size        =  %d" % nx, file=f)
+    print("   MPI tasks           =  %d" % numRanks, file=f)
+    print("   Iteration count     =  %d" % cycle, file=f)
+    print("   Final Origin Energy =  %.6e" % e1, file=f)
+    print("   Testing Plane 0 of Energy Array on rank 0:", file=f)
+    print("        MaxAbsDiff   = %.6e" % maxabs, file=f)
+    print("        TotalAbsDiff = %.6e" % totabs, file=f)
+    print("        MaxRelDiff   = %.6e" % maxrel, file=f)
+    print("\nElapsed time         = %.2f (s)" % elapsed_time, file=f)
+    print("Grind time (us/z/c)  =  %.8f  (per dom)  (%.8f overall)" % (grind1, elapsed_time),
+          file=f)
+    print("FOM                  = %.8f (z/s)\n" % (1000 / grind2), file=f)
+    return {"cycle": cycle, "e1": e1, "MaxAbsDiff": maxabs, "TotalAbsDiff": totabs,
+            "MaxRelDiff": maxrel, "elapsed": elapsed_time}
+
+
+def main(nx, structured=True, num_iters=10, quirks=QUIRKS_REFERENCE, device=0, per_cycle=False,
+         file=None):
+    """examples/benchmark.jl main() for one rank; the cycle loop runs on the
+    device unless per_cycle=True, which steps it from the host exactly as
+    the reference's while-loop does."""
+    prob = LuleshProblem(num_iters, structured, nx, 1, 1, 1)
+    domain = Domain(prob, quirks=quirks, device=device)
+    start = _time.time()
+    if per_cycle:
+        while domain.time < domain.stoptime:
+            timeIncrement(domain)
+            lagrangeLeapFrog(domain)
+            if domain.cycle >= num_iters:
+                break
+    else:
+        run(domain, num_iters)
+    elapsed = _time.time() - start
+    report = verifyAndWriteFinalOutput(elapsed, domain, nx, 1, file=file)
+    domain.close()
+    return report

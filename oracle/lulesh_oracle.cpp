@@ -1,0 +1,1395 @@
+// CPU oracle for the LULESH Lagrange leap-frog cycle -- TEST INFRASTRUCTURE ONLY.
+//
+// Plain C++17 restatement of the reference Julia code, function by function,
+// each citing the file:line it follows.  It exists to check the HIP product
+// path (tests/, the smoke check) and to give bench.py its CPU baseline; no
+// product code links or calls it.
+//
+// Arithmetic rules that make it a faithful restatement of the Julia source:
+//   * built with -ffp-contract=off and without -ffast-math: Julia does not
+//     contract a*b+c into an FMA unless muladd/@fastmath is written, and
+//     neither appears in the reference;
+//   * n-ary Julia sums/products (a+b+c+d, a*b*c) fold left, so every
+//     expression here is written with the same left-to-right association;
+//   * sqrt and / are IEEE correctly rounded on both sides;
+//   * cbrt is correctly rounded here (see oracle_cbrt), while the reference
+//     uses Julia 1.7 Base.Math.cbrt (< 1 ulp, not correctly rounded):
+//     agreement with the reference's cbrt is parity-unpinned at the ulp level;
+//   * the per-node gathers at src/domain.jl:1118 and :1684 carry @simd, which
+//     licenses Julia to reassociate those <=8-term sums; the oracle sums in
+//     list order (ascending element id).  Parity-unpinned at the ulp level.
+//
+// The three arithmetic quirks of the reference (node normal of face 5,
+// volume-derivative slots 2<->4, MonoQ xi+ neighbour) are on by default and
+// can be switched off through the quirk bits of lulesh_oracle.h.
+#include "lulesh_oracle.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// error codes, numbered in reference program order (same numbers as the
+// product's C-ABI header)
+enum {
+  E_EARLY_VOLUME = 1,  // src/domain.jl:1104
+  E_MID_VOLUME = 2,    // src/domain.jl:1758
+  E_HG_VOLUME = 3,     // src/domain.jl:1727
+  E_KIN_VOLUME = 4,    // src/domain.jl:2118
+  E_LAG_VOLUME = 5,    // src/domain.jl:2169
+  E_BC = 6,            // src/domain.jl:2343 (and 2355, 2393, 2405, 2441, 2453)
+  E_QSTOP = 7,         // src/domain.jl:2580
+  E_EOS_VOLUME = 8,    // src/domain.jl:2905
+  E_UPDATE_VOLUME = 9, // src/domain.jl:2926
+};
+
+// src/bc.jl:3-31
+constexpr uint32_t XI_M = 0x00007, XI_M_SYMM = 0x00001, XI_M_FREE = 0x00002, XI_M_COMM = 0x00004;
+constexpr uint32_t XI_P = 0x00038, XI_P_SYMM = 0x00008, XI_P_FREE = 0x00010, XI_P_COMM = 0x00020;
+constexpr uint32_t ETA_M = 0x001c0, ETA_M_SYMM = 0x00040, ETA_M_FREE = 0x00080, ETA_M_COMM = 0x00100;
+constexpr uint32_t ETA_P = 0x00e00, ETA_P_SYMM = 0x00200, ETA_P_FREE = 0x00400, ETA_P_COMM = 0x00800;
+constexpr uint32_t ZETA_M = 0x07000, ZETA_M_SYMM = 0x01000, ZETA_M_FREE = 0x02000, ZETA_M_COMM = 0x04000;
+constexpr uint32_t ZETA_P = 0x38000, ZETA_P_SYMM = 0x08000, ZETA_P_FREE = 0x10000, ZETA_P_COMM = 0x20000;
+
+// Julia 1.7 Base.max/min for Float64 (NaN-propagating, -0 < +0)
+inline double jmax(double x, double y) {
+  bool takey = (y > x) || (std::signbit(y) < std::signbit(x));
+  return takey ? (std::isnan(x) ? x : y) : (std::isnan(y) ? y : x);
+}
+inline double jmin(double x, double y) {
+  bool takey = (y < x) || (std::signbit(y) > std::signbit(x));
+  return takey ? (std::isnan(x) ? x : y) : (std::isnan(y) ? y : x);
+}
+
+// ---------------------------------------------------------------------------
+// correctly rounded cube root.  Seed from libm, then fix the last bit by an
+// exact integer comparison of x against the cube of the rounding midpoints.
+// ---------------------------------------------------------------------------
+typedef unsigned __int128 u128;
+
+struct U192 {
+  uint64_t w[3];  // little endian limbs
+};
+
+inline U192 cube54(uint64_t a) {  // a < 2^55
+  u128 sq = (u128)a * a;          // < 2^110
+  uint64_t lo = (uint64_t)sq, hi = (uint64_t)(sq >> 64);
+  u128 p0 = (u128)lo * a;
+  u128 p1 = (u128)hi * a;
+  U192 r;
+  r.w[0] = (uint64_t)p0;
+  u128 mid = (p0 >> 64) + (uint64_t)p1;
+  r.w[1] = (uint64_t)mid;
+  r.w[2] = (uint64_t)(p1 >> 64) + (uint64_t)(mid >> 64);
+  return r;
+}
+
+inline int cmp192(const U192 &a, const U192 &b) {
+  for (int i = 2; i >= 0; --i) {
+    if (a.w[i] < b.w[i]) return -1;
+    if (a.w[i] > b.w[i]) return 1;
+  }
+  return 0;
+}
+
+double cbrt_cr(double x) {
+  if (x == 0.0 || !std::isfinite(x)) return x + x;  // +-0, +-inf, nan
+  double ax = std::fabs(x);
+  int e2;
+  double m = std::frexp(ax, &e2);  // ax = m * 2^e2, m in [0.5, 1)
+  // integer mantissa M in [2^52, 2^53) with ax = M * 2^(E-52)
+  uint64_t M = (uint64_t)std::ldexp(m, 53);
+  int E = e2 - 1;
+  int q = (E >= 0) ? E / 3 : -((-E + 2) / 3);
+  int r = E - 3 * q;                 // 0, 1, 2
+  uint64_t X = M << r;               // t = X * 2^-52 in [1, 8)
+  double t = std::ldexp((double)X, -52);
+  double y = std::cbrt(t);           // seed, within an ulp or two of cbrt(t) in [1, 2)
+  int64_t K = (int64_t)std::llround(std::ldexp(y, 52));
+  // 8T = X * 2^107 as 192-bit
+  U192 T8;
+  T8.w[0] = 0;
+  T8.w[1] = X << 43;
+  T8.w[2] = X >> 21;
+  for (int it = 0; it < 8; ++it) {
+    U192 up = cube54((uint64_t)(2 * K + 1));
+    if (cmp192(up, T8) < 0) { ++K; continue; }
+    U192 dn = cube54((uint64_t)(2 * K - 1));
+    if (cmp192(dn, T8) > 0) { --K; continue; }
+    break;
+  }
+  double res = std::ldexp((double)K, q - 52);
+  return std::signbit(x) ? -res : res;
+}
+
+// ---------------------------------------------------------------------------
+// element-local functions
+// ---------------------------------------------------------------------------
+
+// src/utils.jl:40-43
+inline double triple_product(double x1, double y1, double z1, double x2, double y2, double z2,
+                             double x3, double y3, double z3) {
+  return ((x1 * ((y2 * z3) - (z2 * y3))) + (x2 * ((z1 * y3) - (y1 * z3)))) +
+         (x3 * ((y1 * z2) - (z1 * y2)));
+}
+
+// src/utils.jl:36-115 (calcElemVolume with the 24 scalars)
+double elem_volume(const double *x, const double *y, const double *z) {
+  const double x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3], x4 = x[4], x5 = x[5], x6 = x[6], x7 = x[7];
+  const double y0 = y[0], y1 = y[1], y2 = y[2], y3 = y[3], y4 = y[4], y5 = y[5], y6 = y[6], y7 = y[7];
+  const double z0 = z[0], z1 = z[1], z2 = z[2], z3 = z[3], z4 = z[4], z5 = z[5], z6 = z[6], z7 = z[7];
+  const double twelveth = 1.0 / 12.0;
+  double dx61 = x6 - x1, dy61 = y6 - y1, dz61 = z6 - z1;
+  double dx70 = x7 - x0, dy70 = y7 - y0, dz70 = z7 - z0;
+  double dx63 = x6 - x3, dy63 = y6 - y3, dz63 = z6 - z3;
+  double dx20 = x2 - x0, dy20 = y2 - y0, dz20 = z2 - z0;
+  double dx50 = x5 - x0, dy50 = y5 - y0, dz50 = z5 - z0;
+  double dx64 = x6 - x4, dy64 = y6 - y4, dz64 = z6 - z4;
+  double dx31 = x3 - x1, dy31 = y3 - y1, dz31 = z3 - z1;
+  double dx72 = x7 - x2, dy72 = y7 - y2, dz72 = z7 - z2;
+  double dx43 = x4 - x3, dy43 = y4 - y3, dz43 = z4 - z3;
+  double dx57 = x5 - x7, dy57 = y5 - y7, dz57 = z5 - z7;
+  double dx14 = x1 - x4, dy14 = y1 - y4, dz14 = z1 - z4;
+  double dx25 = x2 - x5, dy25 = y2 - y5, dz25 = z2 - z5;
+  double volume =
+      (triple_product(dx31 + dx72, dx63, dx20, dy31 + dy72, dy63, dy20, dz31 + dz72, dz63, dz20) +
+       triple_product(dx43 + dx57, dx64, dx70, dy43 + dy57, dy64, dy70, dz43 + dz57, dz64, dz70)) +
+      triple_product(dx14 + dx25, dx61, dx50, dy14 + dy25, dy61, dy50, dz14 + dz25, dz61, dz50);
+  volume = volume * twelveth;
+  return volume;
+}
+
+// src/domain.jl:971-989
+inline void face_normal(double x0, double y0, double z0, double x1, double y1, double z1, double x2,
+                        double y2, double z2, double x3, double y3, double z3, double &ax,
+                        double &ay, double &az) {
+  const double RHALF = 0.5, RQTR = 0.25;
+  double bx0 = RHALF * (((x3 + x2) - x1) - x0);
+  double by0 = RHALF * (((y3 + y2) - y1) - y0);
+  double bz0 = RHALF * (((z3 + z2) - z1) - z0);
+  double bx1 = RHALF * (((x2 + x1) - x3) - x0);
+  double by1 = RHALF * (((y2 + y1) - y3) - y0);
+  double bz1 = RHALF * (((z2 + z1) - z3) - z0);
+  ax = RQTR * ((by0 * bz1) - (bz0 * by1));
+  ay = RQTR * ((bz0 * bx1) - (bx0 * bz1));
+  az = RQTR * ((bx0 * by1) - (by0 * bx1));
+}
+
+// src/domain.jl:991-1067; B[n][c] with c = x,y,z
+void elem_node_normals(const double *x, const double *y, const double *z, int quirks, double B[8][3]) {
+  for (int n = 0; n < 8; ++n) B[n][0] = B[n][1] = B[n][2] = 0.0;
+  double ax, ay, az;
+  auto add = [&](int n) { B[n][0] += ax; B[n][1] += ay; B[n][2] += az; };
+  // face 1: nodes 1,2,3,4
+  face_normal(x[0], y[0], z[0], x[1], y[1], z[1], x[2], y[2], z[2], x[3], y[3], z[3], ax, ay, az);
+  add(0); add(1); add(2); add(3);
+  // face 2: nodes 1,5,6,2
+  face_normal(x[0], y[0], z[0], x[4], y[4], z[4], x[5], y[5], z[5], x[1], y[1], z[1], ax, ay, az);
+  add(0); add(1); add(4); add(5);
+  // face 3: nodes 2,6,7,3
+  face_normal(x[1], y[1], z[1], x[5], y[5], z[5], x[6], y[6], z[6], x[2], y[2], z[2], ax, ay, az);
+  add(1); add(2); add(5); add(6);
+  // face 4: nodes 3,7,8,4
+  face_normal(x[2], y[2], z[2], x[6], y[6], z[6], x[7], y[7], z[7], x[3], y[3], z[3], ax, ay, az);
+  add(2); add(3); add(6); add(7);
+  // face 5: nodes 4,8,5,1
+  face_normal(x[3], y[3], z[3], x[7], y[7], z[7], x[4], y[4], z[4], x[0], y[0], z[0], ax, ay, az);
+  if (quirks & ORACLE_Q1_FACE5_NORMAL) {
+    // src/domain.jl:1047-1055: pfy[1] receives areaY twice, pfz[1] nothing
+    B[0][0] += ax; B[3][0] += ax; B[4][0] += ax; B[7][0] += ax;
+    B[0][1] += ay; B[3][1] += ay; B[4][1] += ay; B[7][1] += ay;
+    B[0][1] += ay;
+    B[3][2] += az; B[4][2] += az; B[7][2] += az;
+  } else {
+    add(0); add(3); add(4); add(7);
+  }
+  // face 6: nodes 5,8,7,6
+  face_normal(x[4], y[4], z[4], x[7], y[7], z[7], x[6], y[6], z[6], x[5], y[5], z[5], ax, ay, az);
+  add(4); add(5); add(6); add(7);
+}
+
+// src/domain.jl:1932-2021; returns the Jacobian volume, fills b[8][3]
+double shape_derivatives(const double *x, const double *y, const double *z, double b[8][3]) {
+  const double x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3], x4 = x[4], x5 = x[5], x6 = x[6], x7 = x[7];
+  const double y0 = y[0], y1 = y[1], y2 = y[2], y3 = y[3], y4 = y[4], y5 = y[5], y6 = y[6], y7 = y[7];
+  const double z0 = z[0], z1 = z[1], z2 = z[2], z3 = z[3], z4 = z[4], z5 = z[5], z6 = z[6], z7 = z[7];
+  double fjxxi = .125 * ((((x6 - x0) + (x5 - x3)) - (x7 - x1)) - (x4 - x2));
+  double fjxet = .125 * ((((x6 - x0) - (x5 - x3)) + (x7 - x1)) - (x4 - x2));
+  double fjxze = .125 * ((((x6 - x0) + (x5 - x3)) + (x7 - x1)) + (x4 - x2));
+  double fjyxi = .125 * ((((y6 - y0) + (y5 - y3)) - (y7 - y1)) - (y4 - y2));
+  double fjyet = .125 * ((((y6 - y0) - (y5 - y3)) + (y7 - y1)) - (y4 - y2));
+  double fjyze = .125 * ((((y6 - y0) + (y5 - y3)) + (y7 - y1)) + (y4 - y2));
+  double fjzxi = .125 * ((((z6 - z0) + (z5 - z3)) - (z7 - z1)) - (z4 - z2));
+  double fjzet = .125 * ((((z6 - z0) - (z5 - z3)) + (z7 - z1)) - (z4 - z2));
+  double fjzze = .125 * ((((z6 - z0) + (z5 - z3)) + (z7 - z1)) + (z4 - z2));
+  double cjxxi = (fjyet * fjzze) - (fjzet * fjyze);
+  double cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
+  double cjxze = (fjyxi * fjzet) - (fjzxi * fjyet);
+  double cjyxi = -(fjxet * fjzze) + (fjzet * fjxze);
+  double cjyet = (fjxxi * fjzze) - (fjzxi * fjxze);
+  double cjyze = -(fjxxi * fjzet) + (fjzxi * fjxet);
+  double cjzxi = (fjxet * fjyze) - (fjyet * fjxze);
+  double cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
+  double cjzze = (fjxxi * fjyet) - (fjyxi * fjxet);
+  b[0][0] = ((-cjxxi) - cjxet) - cjxze;
+  b[1][0] = (cjxxi - cjxet) - cjxze;
+  b[2][0] = (cjxxi + cjxet) - cjxze;
+  b[3][0] = ((-cjxxi) + cjxet) - cjxze;
+  b[4][0] = -b[2][0];
+  b[5][0] = -b[3][0];
+  b[6][0] = -b[0][0];
+  b[7][0] = -b[1][0];
+  b[0][1] = ((-cjyxi) - cjyet) - cjyze;
+  b[1][1] = (cjyxi - cjyet) - cjyze;
+  b[2][1] = (cjyxi + cjyet) - cjyze;
+  b[3][1] = ((-cjyxi) + cjyet) - cjyze;
+  b[4][1] = -b[2][1];
+  b[5][1] = -b[3][1];
+  b[6][1] = -b[0][1];
+  b[7][1] = -b[1][1];
+  b[0][2] = ((-cjzxi) - cjzet) - cjzze;
+  b[1][2] = (cjzxi - cjzet) - cjzze;
+  b[2][2] = (cjzxi + cjzet) - cjzze;
+  b[3][2] = ((-cjzxi) + cjzet) - cjzze;
+  b[4][2] = -b[2][2];
+  b[5][2] = -b[3][2];
+  b[6][2] = -b[0][2];
+  b[7][2] = -b[1][2];
+  return 8.0 * (((fjxet * cjxet) + (fjyet * cjyet)) + (fjzet * cjzet));
+}
+
+// src/domain.jl:1178-1207
+inline void volu_der(double x0, double x1, double x2, double x3, double x4, double x5, double y0,
+                     double y1, double y2, double y3, double y4, double y5, double z0, double z1,
+                     double z2, double z3, double z4, double z5, double &dvdx, double &dvdy,
+                     double &dvdz) {
+  double vx = (((((y1 + y2) * (z0 + z1)) - ((y0 + y1) * (z1 + z2))) + ((y0 + y4) * (z3 + z4))) -
+               ((y3 + y4) * (z0 + z4))) -
+              ((y2 + y5) * (z3 + z5));
+  vx = vx + ((y3 + y5) * (z2 + z5));
+  double vy = ((((-(x1 + x2)) * (z0 + z1)) + ((x0 + x1) * (z1 + z2))) - ((x0 + x4) * (z3 + z4))) +
+              ((x3 + x4) * (z0 + z4));
+  vy = (vy + ((x2 + x5) * (z3 + z5))) - ((x3 + x5) * (z2 + z5));
+  double vz = ((((-(y1 + y2)) * (x0 + x1)) + ((y0 + y1) * (x1 + x2))) - ((y0 + y4) * (x3 + x4))) +
+              ((y3 + y4) * (x0 + x4));
+  vz = (vz + ((y2 + y5) * (x3 + x5))) - ((y3 + y5) * (x2 + x5));
+  const double twelfth = 1.0 / 12.0;
+  dvdx = vx * twelfth;
+  dvdy = vy * twelfth;
+  dvdz = vz * twelfth;
+}
+
+// src/domain.jl:1209-1256.  The reference stores the derivative with
+// respect to node 4 in slot 2 and node 2 in slot 4 (quirk Q2).
+void elem_volume_derivative(const double *x, const double *y, const double *z, int quirks,
+                            double *dvdx, double *dvdy, double *dvdz) {
+  static const int P[8][6] = {{1, 2, 3, 4, 5, 7}, {0, 1, 2, 7, 4, 6}, {3, 0, 1, 6, 7, 5},
+                              {2, 3, 0, 5, 6, 4}, {7, 6, 5, 0, 3, 1}, {4, 7, 6, 1, 0, 2},
+                              {5, 4, 7, 2, 1, 3}, {6, 5, 4, 3, 2, 0}};
+  for (int c = 0; c < 8; ++c) {
+    const int *p = P[c];
+    int slot = c;
+    if (!(quirks & ORACLE_Q2_DVDX_SWAP)) {
+      if (c == 1) slot = 3;
+      else if (c == 3) slot = 1;
+    }
+    volu_der(x[p[0]], x[p[1]], x[p[2]], x[p[3]], x[p[4]], x[p[5]], y[p[0]], y[p[1]], y[p[2]],
+             y[p[3]], y[p[4]], y[p[5]], z[p[0]], z[p[1]], z[p[2]], z[p[3]], z[p[4]], z[p[5]],
+             dvdx[slot], dvdy[slot], dvdz[slot]);
+  }
+}
+
+// src/domain.jl:1490-1499
+const double GAMMA[8][4] = {{1., 1., 1., -1.},  {1., -1., -1., 1.}, {-1., -1., 1., -1.},
+                            {-1., 1., -1., 1.}, {-1., -1., 1., 1.}, {-1., 1., -1., -1.},
+                            {1., 1., 1., 1.},   {1., -1., -1., -1.}};
+
+// src/domain.jl:1505-1553 (hourgam of one element) and 1258-1460 (force)
+void fb_hourglass_elem(const double *x8n, const double *y8n, const double *z8n, const double *dvdx,
+                       const double *dvdy, const double *dvdz, double determ, const double *xd1,
+                       const double *yd1, const double *zd1, double ss1, double mass1, double hourg,
+                       double *hgfx, double *hgfy, double *hgfz) {
+  double hourgam[8][4];
+  double volinv = 1.0 / determ;
+  for (int m = 0; m < 4; ++m) {
+    double hx = x8n[0] * GAMMA[0][m], hy = y8n[0] * GAMMA[0][m], hz = z8n[0] * GAMMA[0][m];
+    for (int n = 1; n < 8; ++n) {
+      hx = hx + x8n[n] * GAMMA[n][m];
+      hy = hy + y8n[n] * GAMMA[n][m];
+      hz = hz + z8n[n] * GAMMA[n][m];
+    }
+    for (int n = 0; n < 8; ++n)
+      hourgam[n][m] =
+          GAMMA[n][m] - volinv * (((dvdx[n] * hx) + (dvdy[n] * hy)) + (dvdz[n] * hz));
+  }
+  double volume13 = cbrt_cr(determ);
+  double coefficient = ((((-hourg) * 0.01) * ss1) * mass1) / volume13;
+  const double *vel[3] = {xd1, yd1, zd1};
+  double *out[3] = {hgfx, hgfy, hgfz};
+  for (int c = 0; c < 3; ++c) {
+    double h[4];
+    for (int m = 0; m < 4; ++m) {
+      double acc = hourgam[0][m] * vel[c][0];
+      for (int n = 1; n < 8; ++n) acc = acc + hourgam[n][m] * vel[c][n];
+      h[m] = acc;
+    }
+    for (int n = 0; n < 8; ++n)
+      out[c][n] = coefficient * ((((hourgam[n][0] * h[0]) + (hourgam[n][1] * h[1])) +
+                                  (hourgam[n][2] * h[2])) +
+                                 (hourgam[n][3] * h[3]));
+  }
+}
+
+// src/domain.jl:1874-1891
+inline double area_face(double x0, double x1, double x2, double x3, double y0, double y1, double y2,
+                        double y3, double z0, double z1, double z2, double z3) {
+  double fx = (x2 - x0) - (x3 - x1);
+  double fy = (y2 - y0) - (y3 - y1);
+  double fz = (z2 - z0) - (z3 - z1);
+  double gx = (x2 - x0) + (x3 - x1);
+  double gy = (y2 - y0) + (y3 - y1);
+  double gz = (z2 - z0) + (z3 - z1);
+  double fg = ((fx * gx) + (fy * gy)) + (fz * gz);
+  return ((((fx * fx) + (fy * fy)) + (fz * fz)) * (((gx * gx) + (gy * gy)) + (gz * gz))) -
+         (fg * fg);
+}
+
+// src/domain.jl:1893-1930
+double char_length(const double *x, const double *y, const double *z, double volume) {
+  static const int F[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                              {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
+  double cl = 0.0;
+  for (int f = 0; f < 6; ++f) {
+    const int *q = F[f];
+    double a = area_face(x[q[0]], x[q[1]], x[q[2]], x[q[3]], y[q[0]], y[q[1]], y[q[2]], y[q[3]],
+                         z[q[0]], z[q[1]], z[q[2]], z[q[3]]);
+    cl = jmax(a, cl);
+  }
+  return (4.0 * volume) / std::sqrt(cl);
+}
+
+// src/domain.jl:2023-2081
+void velocity_gradient(const double *xv, const double *yv, const double *zv, const double b[8][3],
+                       double detJ, double d[6]) {
+  double inv_detJ = 1.0 / detJ;
+  auto g = [&](int c, const double *v) {
+    return (((b[0][c] * (v[0] - v[6])) + (b[1][c] * (v[1] - v[7]))) + (b[2][c] * (v[2] - v[4]))) +
+           (b[3][c] * (v[3] - v[5]));
+  };
+  d[0] = inv_detJ * g(0, xv);
+  d[1] = inv_detJ * g(1, yv);
+  d[2] = inv_detJ * g(2, zv);
+  double dyddx = inv_detJ * g(0, yv);
+  double dxddy = inv_detJ * g(1, xv);
+  double dzddx = inv_detJ * g(0, zv);
+  double dxddz = inv_detJ * g(2, xv);
+  double dzddy = inv_detJ * g(1, zv);
+  double dyddz = inv_detJ * g(2, yv);
+  d[5] = 0.5 * (dxddy + dyddx);
+  d[4] = 0.5 * (dxddz + dzddx);
+  d[3] = 0.5 * (dzddy + dyddz);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Domain
+// ---------------------------------------------------------------------------
+struct oracle_domain {
+  int quirks = ORACLE_QUIRKS_REFERENCE;
+  int nthreads = 1;
+  int64_t sizeX = 0, numElem = 0, numNode = 0;
+  // element
+  std::vector<int64_t> matElemlist, nodelist, lxim, lxip, letam, letap, lzetam, lzetap;
+  std::vector<uint32_t> elemBC;
+  std::vector<double> e, p, q, ql, qq, v, volo, delv, vdov, arealg, ss, elemMass;
+  std::vector<double> vnew, delv_xi, delv_eta, delv_zeta, delx_xi, delx_eta, delx_zeta, dxx, dyy, dzz;
+  // node
+  std::vector<double> x, y, z, xd, yd, zd, xdd, ydd, zdd, fx, fy, fz, nodalMass;
+  std::vector<int64_t> symmX, symmY, symmZ, nodeElemCount, nodeElemStart, nodeElemCornerList;
+  // scratch (the reference allocates these per call)
+  std::vector<double> sig, determ, fx_elem, fy_elem, fz_elem;
+  std::vector<double> dvdx, dvdy, dvdz, x8n, y8n, z8n;
+  // scalars (src/types.jl:91-126)
+  double dtfixed = 0, deltatimemultlb = 0, deltatimemultub = 0, stoptime = 0, dtmax = 0;
+  int64_t cycle = 0;
+  double dthydro = 0, dtcourant = 0, time = 0, deltatime = 0;
+  double u_cut = 0, hgcoef = 0, qstop = 0, monoq_max_slope = 0, monoq_limiter_mult = 0;
+  double e_cut = 0, p_cut = 0, ss4o3 = 0, q_cut = 0, v_cut = 0, qlc_monoq = 0, qqc_monoq = 0;
+  double qqc = 0, eosvmax = 0, eosvmin = 0, pmin = 0, emin = 0, dvovmax = 0, refdens = 0;
+};
+
+#define OMP_FOR _Pragma("omp parallel for schedule(static) num_threads(d->nthreads) if(d->nthreads > 1)")
+
+namespace {
+
+// src/domain.jl:117-174
+void build_mesh(oracle_domain *d, int64_t nx) {
+  const int64_t edgeNodes = nx + 1, edgeElems = nx;
+  const int64_t meshEdgeElems = 1 * nx;  // m_tp = 1
+  int64_t nidx = 0;
+  double tz = 1.125 * (double)(0 * nx) / (double)meshEdgeElems;
+  for (int64_t plane = 1; plane <= edgeNodes; ++plane) {
+    double ty = 1.125 * (double)(0 * nx) / (double)meshEdgeElems;
+    for (int64_t row = 1; row <= edgeNodes; ++row) {
+      double tx = 1.125 * (double)(0 * nx) / (double)meshEdgeElems;
+      for (int64_t col = 1; col <= edgeNodes; ++col) {
+        d->x[nidx] = tx;
+        d->y[nidx] = ty;
+        d->z[nidx] = tz;
+        ++nidx;
+        tx = (1.125 * (double)(0 * nx + col)) / (double)meshEdgeElems;
+      }
+      ty = (1.125 * (double)(0 * nx + row)) / (double)meshEdgeElems;
+    }
+    tz = (1.125 * (double)(0 * nx + plane)) / (double)meshEdgeElems;
+  }
+  int64_t zidx = 0;
+  nidx = 0;
+  for (int64_t plane = 0; plane < edgeElems; ++plane) {
+    for (int64_t row = 0; row < edgeElems; ++row) {
+      for (int64_t col = 0; col < edgeElems; ++col) {
+        int64_t *nl = &d->nodelist[8 * zidx];
+        nl[0] = nidx;
+        nl[1] = nidx + 1;
+        nl[2] = nidx + edgeNodes + 1;
+        nl[3] = nidx + edgeNodes;
+        nl[4] = nidx + edgeNodes * edgeNodes;
+        nl[5] = nidx + edgeNodes * edgeNodes + 1;
+        nl[6] = nidx + edgeNodes * edgeNodes + edgeNodes + 1;
+        nl[7] = nidx + edgeNodes * edgeNodes + edgeNodes;
+        ++zidx;
+        ++nidx;
+      }
+      ++nidx;
+    }
+    nidx += edgeNodes;
+  }
+}
+
+// src/domain.jl:773-791
+void setup_symmetry_planes(oracle_domain *d, int64_t edgeNodes) {
+  int64_t nidx = 0;
+  for (int64_t i = 0; i < edgeNodes; ++i) {
+    int64_t planeInc = i * edgeNodes * edgeNodes, rowInc = i * edgeNodes;
+    for (int64_t j = 0; j < edgeNodes; ++j) {
+      d->symmZ[nidx] = rowInc + j;
+      d->symmY[nidx] = planeInc + j;
+      d->symmX[nidx] = planeInc + j * edgeNodes;
+      ++nidx;
+    }
+  }
+}
+
+// src/domain.jl:793-820 (stored values are 0-based element ids; the
+// reference's first xi-minus entry is 1, and the last eta-plus row stores i)
+void setup_element_connectivities(oracle_domain *d, int64_t edgeElems) {
+  const int64_t E = d->numElem;
+  d->lxim[0] = 1;
+  for (int64_t i = 1; i <= E - 1; ++i) {
+    d->lxim[i] = i - 1;
+    d->lxip[i - 1] = i;
+  }
+  d->lxip[E - 1] = E - 1;
+  for (int64_t i = 0; i < edgeElems; ++i) {
+    d->letam[i] = i;
+    d->letap[E - edgeElems + i] = i;
+  }
+  for (int64_t i = edgeElems; i <= E - 1; ++i) {
+    d->letam[i] = i - edgeElems;
+    d->letap[i - edgeElems] = i;
+  }
+  for (int64_t i = 0; i < edgeElems * edgeElems; ++i) {
+    d->lzetam[i] = i;
+    d->lzetap[E - edgeElems * edgeElems + i] = E - edgeElems * edgeElems + i;
+  }
+  for (int64_t i = edgeElems * edgeElems; i <= E - 1; ++i) {
+    d->lzetam[i] = i - edgeElems * edgeElems;
+    d->lzetap[i - edgeElems * edgeElems] = i;
+  }
+}
+
+// src/domain.jl:822-916 for the single rank (all min faces SYMM, all max
+// faces FREE; no ghost indices are written)
+void setup_boundary_conditions(oracle_domain *d, int64_t edgeElems) {
+  const int64_t E = d->numElem;
+  for (int64_t i = 0; i < E; ++i) d->elemBC[i] = 0;
+  for (int64_t i = 1; i <= edgeElems; ++i) {
+    int64_t planeInc = (i - 1) * edgeElems * edgeElems, rowInc = (i - 1) * edgeElems;
+    for (int64_t j = 1; j <= edgeElems; ++j) {
+      d->elemBC[rowInc + j - 1] |= ZETA_M_SYMM;
+      d->elemBC[rowInc + j + E - edgeElems * edgeElems - 1] |= ZETA_P_FREE;
+      d->elemBC[planeInc + j - 1] |= ETA_M_SYMM;
+      d->elemBC[planeInc + j + edgeElems * edgeElems - edgeElems - 1] |= ETA_P_FREE;
+      d->elemBC[planeInc + (j - 1) * edgeElems] |= XI_M_SYMM;
+      d->elemBC[planeInc + (j - 1) * edgeElems + edgeElems - 1] |= XI_P_FREE;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" oracle_domain *oracle_create(int nx, int quirks, int nthreads) {
+  if (nx < 1) return nullptr;
+  oracle_domain *d = new oracle_domain();
+  d->quirks = quirks;
+  d->nthreads = nthreads < 1 ? 1 : nthreads;
+  const int64_t edgeElems = nx, edgeNodes = nx + 1;
+  d->sizeX = nx;
+  d->numElem = (int64_t)nx * nx * nx;
+  d->numNode = edgeNodes * edgeNodes * edgeNodes;
+  const int64_t E = d->numElem, N = d->numNode;
+  // allocateElemPersistent!/allocateNodalPersistent!/initializeFields!
+  d->matElemlist.resize(E);
+  d->nodelist.assign(8 * E, 0);
+  d->lxim.resize(E); d->lxip.resize(E); d->letam.resize(E); d->letap.resize(E);
+  d->lzetam.resize(E); d->lzetap.resize(E);
+  d->elemBC.assign(E, 0);
+  d->e.assign(E, 0.0); d->p.assign(E, 0.0); d->q.assign(E, 0.0);
+  d->ql.assign(E, 0.0); d->qq.assign(E, 0.0); d->v.assign(E, 1.0);
+  d->volo.assign(E, 0.0); d->delv.assign(E, 0.0); d->vdov.assign(E, 0.0);
+  d->arealg.assign(E, 0.0); d->ss.assign(E, 0.0); d->elemMass.assign(E, 0.0);
+  d->vnew.assign(E, 0.0); d->dxx.assign(E, 0.0); d->dyy.assign(E, 0.0); d->dzz.assign(E, 0.0);
+  d->delx_xi.assign(E, 0.0); d->delx_eta.assign(E, 0.0); d->delx_zeta.assign(E, 0.0);
+  // delv_* hold E + ghost planes in the reference (src/domain.jl:2944-2951);
+  // the ghost part is never read for one domain
+  d->delv_xi.assign(E, 0.0); d->delv_eta.assign(E, 0.0); d->delv_zeta.assign(E, 0.0);
+  d->x.assign(N, 0.0); d->y.assign(N, 0.0); d->z.assign(N, 0.0);
+  d->xd.assign(N, 0.0); d->yd.assign(N, 0.0); d->zd.assign(N, 0.0);
+  d->xdd.assign(N, 0.0); d->ydd.assign(N, 0.0); d->zdd.assign(N, 0.0);
+  d->fx.assign(N, 0.0); d->fy.assign(N, 0.0); d->fz.assign(N, 0.0);
+  d->nodalMass.assign(N, 0.0);
+  d->symmX.assign(edgeNodes * edgeNodes, 0);
+  d->symmY.assign(edgeNodes * edgeNodes, 0);
+  d->symmZ.assign(edgeNodes * edgeNodes, 0);
+
+  build_mesh(d, nx);
+  setup_symmetry_planes(d, edgeNodes);
+
+  // node -> corner CSR, src/domain.jl:615-654 (corner ids stored 1-based)
+  d->nodeElemCount.assign(N, 0);
+  for (int64_t i = 0; i < E; ++i)
+    for (int j = 0; j < 8; ++j) d->nodeElemCount[d->nodelist[8 * i + j]] += 1;
+  d->nodeElemStart.assign(N, 0);
+  for (int64_t i = 1; i < N; ++i) d->nodeElemStart[i] = d->nodeElemStart[i - 1] + d->nodeElemCount[i - 1];
+  d->nodeElemCornerList.assign(d->nodeElemStart[N - 1] + d->nodeElemCount[N - 1], 0);
+  std::fill(d->nodeElemCount.begin(), d->nodeElemCount.end(), 0);
+  for (int64_t i = 0; i < E; ++i)
+    for (int j = 0; j < 8; ++j) {
+      int64_t m = d->nodelist[8 * i + j];
+      int64_t k = 8 * i + j + 1;
+      d->nodeElemCornerList[d->nodeElemStart[m] + d->nodeElemCount[m]] = k;
+      d->nodeElemCount[m] += 1;
+    }
+  for (int64_t i = 0; i < E; ++i) d->matElemlist[i] = i + 1;
+
+  // src/domain.jl:663-701
+  d->dthydro = 1e20;
+  d->dtcourant = 1e20;
+  d->time = 0.;
+  d->dtfixed = -1.0e-6;
+  d->deltatimemultlb = 1.1;
+  d->deltatimemultub = 1.2;
+  d->stoptime = 1.0e-2;
+  d->dtmax = 1.0e-2;
+  d->cycle = 0;
+  d->e_cut = 1.0e-7;
+  d->p_cut = 1.0e-7;
+  d->q_cut = 1.0e-7;
+  d->u_cut = 1.0e-7;
+  d->v_cut = 1.0e-10;
+  d->hgcoef = 3.0;
+  d->ss4o3 = 4.0 / 3.0;
+  d->qstop = 1.0e+12;
+  d->monoq_max_slope = 1.0;
+  d->monoq_limiter_mult = 2.0;
+  d->qlc_monoq = 0.5;
+  d->qqc_monoq = 2.0 / 3.0;
+  d->qqc = 2.0;
+  d->pmin = 0.;
+  d->emin = -1.0e+15;
+  d->dvovmax = 0.1;
+  d->eosvmax = 1.0e+9;
+  d->eosvmin = 1.0e-9;
+  d->refdens = 1.0;
+
+  // volumes and masses, src/domain.jl:704-731 (nodal mass in element order)
+  for (int64_t i = 0; i < E; ++i) {
+    double xl[8], yl[8], zl[8];
+    for (int n = 0; n < 8; ++n) {
+      int64_t g = d->nodelist[8 * i + n];
+      xl[n] = d->x[g]; yl[n] = d->y[g]; zl[n] = d->z[g];
+    }
+    double volume = elem_volume(xl, yl, zl);
+    d->volo[i] = volume;
+    d->elemMass[i] = volume;
+    for (int n = 0; n < 8; ++n) {
+      int64_t g = d->nodelist[8 * i + n];
+      d->nodalMass[g] = d->nodalMass[g] + volume / 8.0;
+    }
+  }
+  // energy deposit, src/domain.jl:738-749
+  const double ebase = 3.948746e+7;
+  const double scale = (double)(nx * 1) / 45.0;
+  const double einit = ((ebase * scale) * scale) * scale;
+  d->e[0] = einit;
+  d->deltatime = (.5 * cbrt_cr(d->volo[0])) / std::sqrt(2.0 * einit);
+
+  setup_element_connectivities(d, edgeElems);
+  setup_boundary_conditions(d, edgeElems);
+
+  const int64_t E8 = 8 * E;
+  d->sig.assign(E, 0.0);
+  d->determ.assign(E, 0.0);
+  d->fx_elem.assign(E8, 0.0); d->fy_elem.assign(E8, 0.0); d->fz_elem.assign(E8, 0.0);
+  d->dvdx.assign(E8, 0.0); d->dvdy.assign(E8, 0.0); d->dvdz.assign(E8, 0.0);
+  d->x8n.assign(E8, 0.0); d->y8n.assign(E8, 0.0); d->z8n.assign(E8, 0.0);
+  return d;
+}
+
+extern "C" void oracle_destroy(oracle_domain *d) { delete d; }
+
+// src/domain.jl:920-961
+extern "C" void oracle_time_increment_scalars(double *st) {
+  double time = st[0], deltatime = st[1], cycle = st[2], dtcourant = st[3], dthydro = st[4];
+  const double stoptime = st[5], dtmax = st[6], dtfixed = st[7], lb = st[8], ub = st[9];
+  double targetdt = stoptime - time;
+  if (dtfixed <= 0.0 && cycle != 0) {
+    double olddt = deltatime;
+    double gnewdt = INFINITY;
+    double newdt;
+    if (dtcourant < gnewdt) gnewdt = dtcourant / 2.0;
+    if (dthydro < gnewdt) gnewdt = (dthydro * 2.0) / 3.0;
+    newdt = gnewdt;  // comm_min(x, nothing) == x
+    double ratio = newdt / olddt;
+    if (ratio >= 1.0) {
+      if (ratio < lb) newdt = olddt;
+      else if (ratio > ub) newdt = olddt * ub;
+    }
+    newdt = jmin(newdt, dtmax);
+    deltatime = newdt;
+  }
+  if (deltatime < targetdt && targetdt < (4.0 * deltatime) / 3.0) targetdt = (2.0 * deltatime) / 3.0;
+  if (targetdt < deltatime) deltatime = targetdt;
+  time = time + deltatime;
+  cycle = cycle + 1;
+  st[0] = time; st[1] = deltatime; st[2] = cycle; st[3] = dtcourant; st[4] = dthydro;
+}
+
+extern "C" int oracle_time_increment(oracle_domain *d) {
+  double st[10] = {d->time, d->deltatime, (double)d->cycle, d->dtcourant, d->dthydro,
+                   d->stoptime, d->dtmax, d->dtfixed, d->deltatimemultlb, d->deltatimemultub};
+  oracle_time_increment_scalars(st);
+  d->time = st[0];
+  d->deltatime = st[1];
+  d->cycle = (int64_t)st[2];
+  return 0;
+}
+
+namespace {
+
+inline void gather8(const oracle_domain *d, const std::vector<double> &src, int64_t k, double *out) {
+  for (int n = 0; n < 8; ++n) out[n] = src[d->nodelist[8 * k + n]];
+}
+
+// src/domain.jl:963-969, 1087-1128, 1738-1760
+int integrate_stress(oracle_domain *d) {
+  const int64_t E = d->numElem, N = d->numNode;
+  for (int64_t i = 0; i < E; ++i) d->sig[i] = (-d->p[i]) - d->q[i];
+  int err = 0;
+  OMP_FOR
+  for (int64_t k = 0; k < E; ++k) {
+    double xl[8], yl[8], zl[8], b[8][3], B[8][3];
+    gather8(d, d->x, k, xl); gather8(d, d->y, k, yl); gather8(d, d->z, k, zl);
+    double detJ = shape_derivatives(xl, yl, zl, b);
+    d->determ[k] = detJ;
+    if (d->determ[k] <= 0.0) {
+#pragma omp atomic write
+      err = E_EARLY_VOLUME;
+    }
+    elem_node_normals(xl, yl, zl, d->quirks, B);
+    const double ms = -d->sig[k];
+    for (int n = 0; n < 8; ++n) {
+      d->fx_elem[8 * k + n] = ms * B[n][0];
+      d->fy_elem[8 * k + n] = ms * B[n][1];
+      d->fz_elem[8 * k + n] = ms * B[n][2];
+    }
+  }
+  if (err) return err;
+  OMP_FOR
+  for (int64_t g = 0; g < N; ++g) {
+    const int64_t count = d->nodeElemCount[g], start = d->nodeElemStart[g];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int64_t i = 0; i < count; ++i) {
+      int64_t c = d->nodeElemCornerList[start + i] - 1;
+      fx = fx + d->fx_elem[c];
+      fy = fy + d->fy_elem[c];
+      fz = fz + d->fz_elem[c];
+    }
+    d->fx[g] = fx;
+    d->fy[g] = fy;
+    d->fz[g] = fz;
+  }
+  for (int64_t i = 0; i < E; ++i)
+    if (d->determ[i] <= 0.0) return E_MID_VOLUME;
+  return 0;
+}
+
+// src/domain.jl:1696-1736 and 1462-1694
+int hourglass_control(oracle_domain *d) {
+  const int64_t E = d->numElem, N = d->numNode;
+  int err = 0;
+  OMP_FOR
+  for (int64_t i = 0; i < E; ++i) {
+    double xl[8], yl[8], zl[8];
+    gather8(d, d->x, i, xl); gather8(d, d->y, i, yl); gather8(d, d->z, i, zl);
+    elem_volume_derivative(xl, yl, zl, d->quirks, &d->dvdx[8 * i], &d->dvdy[8 * i], &d->dvdz[8 * i]);
+    for (int n = 0; n < 8; ++n) {
+      d->x8n[8 * i + n] = xl[n];
+      d->y8n[8 * i + n] = yl[n];
+      d->z8n[8 * i + n] = zl[n];
+    }
+    d->determ[i] = d->volo[i] * d->v[i];
+    if (d->v[i] <= 0.0) {
+#pragma omp atomic write
+      err = E_HG_VOLUME;
+    }
+  }
+  if (err) return err;
+  if (!(d->hgcoef > 0.0)) return 0;
+  OMP_FOR
+  for (int64_t i = 0; i < E; ++i) {
+    double xd1[8], yd1[8], zd1[8];
+    gather8(d, d->xd, i, xd1); gather8(d, d->yd, i, yd1); gather8(d, d->zd, i, zd1);
+    fb_hourglass_elem(&d->x8n[8 * i], &d->y8n[8 * i], &d->z8n[8 * i], &d->dvdx[8 * i],
+                      &d->dvdy[8 * i], &d->dvdz[8 * i], d->determ[i], xd1, yd1, zd1, d->ss[i],
+                      d->elemMass[i], d->hgcoef, &d->fx_elem[8 * i], &d->fy_elem[8 * i],
+                      &d->fz_elem[8 * i]);
+  }
+  OMP_FOR
+  for (int64_t g = 0; g < N; ++g) {
+    const int64_t count = d->nodeElemCount[g], start = d->nodeElemStart[g];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int64_t i = 0; i < count; ++i) {
+      int64_t c = d->nodeElemCornerList[start + i] - 1;
+      fx = fx + d->fx_elem[c];
+      fy = fy + d->fy_elem[c];
+      fz = fz + d->fz_elem[c];
+    }
+    d->fx[g] = d->fx[g] + fx;
+    d->fy[g] = d->fy[g] + fy;
+    d->fz[g] = d->fz[g] + fz;
+  }
+  return 0;
+}
+
+}  // namespace
+
+// src/domain.jl:1840-1872 (with 1764-1838)
+extern "C" int oracle_lagrange_nodal(oracle_domain *d) {
+  const double delt = d->deltatime, u_cut = d->u_cut;
+  const int64_t N = d->numNode;
+  std::fill(d->fx.begin(), d->fx.end(), 0.0);
+  std::fill(d->fy.begin(), d->fy.end(), 0.0);
+  std::fill(d->fz.begin(), d->fz.end(), 0.0);
+  int err = integrate_stress(d);
+  if (err) return err;
+  err = hourglass_control(d);
+  if (err) return err;
+  OMP_FOR
+  for (int64_t i = 0; i < N; ++i) {
+    d->xdd[i] = d->fx[i] / d->nodalMass[i];
+    d->ydd[i] = d->fy[i] / d->nodalMass[i];
+    d->zdd[i] = d->fz[i] / d->nodalMass[i];
+  }
+  const int64_t numNodeBC = (d->sizeX + 1) * (d->sizeX + 1);
+  if (!d->symmX.empty())
+    for (int64_t i = 0; i < numNodeBC; ++i) d->xdd[d->symmX[i]] = 0.0;
+  if (!d->symmY.empty())
+    for (int64_t i = 0; i < numNodeBC; ++i) d->ydd[d->symmY[i]] = 0.0;
+  if (!d->symmZ.empty())
+    for (int64_t i = 0; i < numNodeBC; ++i) d->zdd[d->symmZ[i]] = 0.0;
+  OMP_FOR
+  for (int64_t i = 0; i < N; ++i) {
+    double t = d->xd[i] + d->xdd[i] * delt;
+    if (std::fabs(t) < u_cut) t = 0.0;
+    d->xd[i] = t;
+    t = d->yd[i] + d->ydd[i] * delt;
+    if (std::fabs(t) < u_cut) t = 0.0;
+    d->yd[i] = t;
+    t = d->zd[i] + d->zdd[i] * delt;
+    if (std::fabs(t) < u_cut) t = 0.0;
+    d->zd[i] = t;
+  }
+  OMP_FOR
+  for (int64_t i = 0; i < N; ++i) {
+    d->x[i] = d->x[i] + d->xd[i] * delt;
+    d->y[i] = d->y[i] + d->yd[i] * delt;
+    d->z[i] = d->z[i] + d->zd[i] * delt;
+  }
+  return 0;
+}
+
+namespace {
+
+// src/domain.jl:2099-2173
+int lagrange_kinematics(oracle_domain *d, double dt) {
+  const int64_t E = d->numElem;
+  int err = 0;
+  OMP_FOR
+  for (int64_t k = 0; k < E; ++k) {
+    double xl[8], yl[8], zl[8], xv[8], yv[8], zv[8], b[8][3], D[6];
+    gather8(d, d->x, k, xl); gather8(d, d->y, k, yl); gather8(d, d->z, k, zl);
+    double volume = elem_volume(xl, yl, zl);
+    double relativeVolume = volume / d->volo[k];
+    d->vnew[k] = relativeVolume;
+    if (d->vnew[k] <= 0.0) {
+#pragma omp atomic write
+      err = E_KIN_VOLUME;
+    }
+    d->delv[k] = relativeVolume - d->v[k];
+    d->arealg[k] = char_length(xl, yl, zl, volume);
+    gather8(d, d->xd, k, xv); gather8(d, d->yd, k, yv); gather8(d, d->zd, k, zv);
+    double dt2 = 0.5 * dt;
+    for (int n = 0; n < 8; ++n) {
+      xl[n] = xl[n] - dt2 * xv[n];
+      yl[n] = yl[n] - dt2 * yv[n];
+      zl[n] = zl[n] - dt2 * zv[n];
+    }
+    double detJ = shape_derivatives(xl, yl, zl, b);
+    velocity_gradient(xv, yv, zv, b, detJ, D);
+    d->dxx[k] = D[0];
+    d->dyy[k] = D[1];
+    d->dzz[k] = D[2];
+  }
+  if (err) return err;
+  OMP_FOR
+  for (int64_t k = 0; k < E; ++k) {
+    double vdov = (d->dxx[k] + d->dyy[k]) + d->dzz[k];
+    double vdovthird = vdov / 3.0;
+    d->vdov[k] = vdov;
+    d->dxx[k] = d->dxx[k] - vdovthird;
+    d->dyy[k] = d->dyy[k] - vdovthird;
+    d->dzz[k] = d->dzz[k] - vdovthird;
+    if (d->vnew[k] <= 0.0) {
+#pragma omp atomic write
+      err = E_LAG_VOLUME;
+    }
+  }
+  return err;
+}
+
+inline double sum4(double a, double b, double c, double e) { return ((a + b) + c) + e; }
+
+// src/domain.jl:2175-2320
+void monoq_gradients(oracle_domain *d) {
+  const double ptiny = 1.e-36;
+  const int64_t E = d->numElem;
+  OMP_FOR
+  for (int64_t i = 0; i < E; ++i) {
+    double X[8], Y[8], Z[8], XV[8], YV[8], ZV[8];
+    gather8(d, d->x, i, X); gather8(d, d->y, i, Y); gather8(d, d->z, i, Z);
+    gather8(d, d->xd, i, XV); gather8(d, d->yd, i, YV); gather8(d, d->zd, i, ZV);
+    const double vol = d->volo[i] * d->vnew[i];
+    const double norm = 1.0 / (vol + ptiny);
+    double dxj = -0.25 * (sum4(X[0], X[1], X[5], X[4]) - sum4(X[3], X[2], X[6], X[7]));
+    double dyj = -0.25 * (sum4(Y[0], Y[1], Y[5], Y[4]) - sum4(Y[3], Y[2], Y[6], Y[7]));
+    double dzj = -0.25 * (sum4(Z[0], Z[1], Z[5], Z[4]) - sum4(Z[3], Z[2], Z[6], Z[7]));
+    double dxi = 0.25 * (sum4(X[1], X[2], X[6], X[5]) - sum4(X[0], X[3], X[7], X[4]));
+    double dyi = 0.25 * (sum4(Y[1], Y[2], Y[6], Y[5]) - sum4(Y[0], Y[3], Y[7], Y[4]));
+    double dzi = 0.25 * (sum4(Z[1], Z[2], Z[6], Z[5]) - sum4(Z[0], Z[3], Z[7], Z[4]));
+    double dxk = 0.25 * (sum4(X[4], X[5], X[6], X[7]) - sum4(X[0], X[1], X[2], X[3]));
+    double dyk = 0.25 * (sum4(Y[4], Y[5], Y[6], Y[7]) - sum4(Y[0], Y[1], Y[2], Y[3]));
+    double dzk = 0.25 * (sum4(Z[4], Z[5], Z[6], Z[7]) - sum4(Z[0], Z[1], Z[2], Z[3]));
+    double ax, ay, az, dxv, dyv, dzv;
+    // zeta: i cross j
+    ax = (dyi * dzj) - (dzi * dyj);
+    ay = (dzi * dxj) - (dxi * dzj);
+    az = (dxi * dyj) - (dyi * dxj);
+    d->delx_zeta[i] = vol / std::sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+    ax = ax * norm; ay = ay * norm; az = az * norm;
+    dxv = 0.25 * (sum4(XV[4], XV[5], XV[6], XV[7]) - sum4(XV[0], XV[1], XV[2], XV[3]));
+    dyv = 0.25 * (sum4(YV[4], YV[5], YV[6], YV[7]) - sum4(YV[0], YV[1], YV[2], YV[3]));
+    dzv = 0.25 * (sum4(ZV[4], ZV[5], ZV[6], ZV[7]) - sum4(ZV[0], ZV[1], ZV[2], ZV[3]));
+    d->delv_zeta[i] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+    // xi: j cross k
+    ax = (dyj * dzk) - (dzj * dyk);
+    ay = (dzj * dxk) - (dxj * dzk);
+    az = (dxj * dyk) - (dyj * dxk);
+    d->delx_xi[i] = vol / std::sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+    ax = ax * norm; ay = ay * norm; az = az * norm;
+    dxv = 0.25 * (sum4(XV[1], XV[2], XV[6], XV[5]) - sum4(XV[0], XV[3], XV[7], XV[4]));
+    dyv = 0.25 * (sum4(YV[1], YV[2], YV[6], YV[5]) - sum4(YV[0], YV[3], YV[7], YV[4]));
+    dzv = 0.25 * (sum4(ZV[1], ZV[2], ZV[6], ZV[5]) - sum4(ZV[0], ZV[3], ZV[7], ZV[4]));
+    d->delv_xi[i] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+    // eta: k cross i
+    ax = (dyk * dzi) - (dzk * dyi);
+    ay = (dzk * dxi) - (dxk * dzi);
+    az = (dxk * dyi) - (dyk * dxi);
+    d->delx_eta[i] = vol / std::sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+    ax = ax * norm; ay = ay * norm; az = az * norm;
+    dxv = -0.25 * (sum4(XV[0], XV[1], XV[5], XV[4]) - sum4(XV[3], XV[2], XV[6], XV[7]));
+    dyv = -0.25 * (sum4(YV[0], YV[1], YV[5], YV[4]) - sum4(YV[3], YV[2], YV[6], YV[7]));
+    dzv = -0.25 * (sum4(ZV[0], ZV[1], ZV[5], ZV[4]) - sum4(ZV[3], ZV[2], ZV[6], ZV[7]));
+    d->delv_eta[i] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+  }
+}
+
+// limiter of src/domain.jl:2359-2378 (same for all three directions)
+inline double limit_phi(double delvm, double delvp, double norm, double mult, double maxslope) {
+  delvm = delvm * norm;
+  delvp = delvp * norm;
+  double phi = 0.5 * (delvm + delvp);
+  delvm = delvm * mult;
+  delvp = delvp * mult;
+  if (delvm < phi) phi = delvm;
+  if (delvp < phi) phi = delvp;
+  if (phi < 0.0) phi = 0.0;
+  if (phi > maxslope) phi = maxslope;
+  return phi;
+}
+
+// src/domain.jl:2322-2513 (+ 2516-2537)
+int monoq_region(oracle_domain *d) {
+  const double ptiny = 1e-36;
+  const double mult = d->monoq_limiter_mult, maxslope = d->monoq_max_slope;
+  const double qlc = d->qlc_monoq, qqc = d->qqc_monoq;
+  const bool q3 = (d->quirks & ORACLE_Q3_XIP_SELF) != 0;
+  const int64_t E = d->numElem;
+  int err = 0;
+  OMP_FOR
+  for (int64_t ie = 0; ie < E; ++ie) {
+    const int64_t i = d->matElemlist[ie] - 1;
+    const uint32_t bcMask = d->elemBC[i];
+    double delvm = 0.0, delvp = 0.0;
+    bool bad = false;
+    // xi
+    double norm = 1.0 / (d->delv_xi[i] + ptiny);
+    uint32_t c = bcMask & XI_M;
+    if (c == 0 || c == XI_M_COMM) delvm = d->delv_xi[d->lxim[i]];
+    else if (c == XI_M_SYMM) delvm = d->delv_xi[i];
+    else if (c == XI_M_FREE) delvm = 0.0;
+    else bad = true;
+    c = bcMask & XI_P;
+    if (c == 0 || c == XI_P_COMM) delvp = q3 ? d->delv_xi[d->lxip[i] - 1] : d->delv_xi[d->lxip[i]];
+    else if (c == XI_P_SYMM) delvp = d->delv_xi[i];
+    else if (c == XI_P_FREE) delvp = 0.0;
+    else bad = true;
+    const double phixi = limit_phi(delvm, delvp, norm, mult, maxslope);
+    // eta
+    norm = 1.0 / (d->delv_eta[i] + ptiny);
+    c = bcMask & ETA_M;
+    if (c == 0 || c == ETA_M_COMM) delvm = d->delv_eta[d->letam[i]];
+    else if (c == ETA_M_SYMM) delvm = d->delv_eta[i];
+    else if (c == ETA_M_FREE) delvm = 0.0;
+    else { delvm = 0.0; bad = true; }
+    c = bcMask & ETA_P;
+    if (c == 0 || c == ETA_P_COMM) delvp = d->delv_eta[d->letap[i]];
+    else if (c == ETA_P_SYMM) delvp = d->delv_eta[i];
+    else if (c == ETA_P_FREE) delvp = 0.0;
+    else { delvp = 0.0; bad = true; }
+    const double phieta = limit_phi(delvm, delvp, norm, mult, maxslope);
+    // zeta
+    norm = 1.0 / (d->delv_zeta[i] + ptiny);
+    c = bcMask & ZETA_M;
+    if (c == 0 || c == ZETA_M_COMM) delvm = d->delv_zeta[d->lzetam[i]];
+    else if (c == ZETA_M_SYMM) delvm = d->delv_zeta[i];
+    else if (c == ZETA_M_FREE) delvm = 0.0;
+    else { delvm = 0.0; bad = true; }
+    c = bcMask & ZETA_P;
+    if (c == 0 || c == ZETA_P_COMM) delvp = d->delv_zeta[d->lzetap[i]];
+    else if (c == ZETA_P_SYMM) delvp = d->delv_zeta[i];
+    else if (c == ZETA_P_FREE) delvp = 0.0;
+    else { delvp = 0.0; bad = true; }
+    const double phizeta = limit_phi(delvm, delvp, norm, mult, maxslope);
+    if (bad) {
+#pragma omp atomic write
+      err = E_BC;
+    }
+    double qlin, qquad;
+    if (d->vdov[i] > 0.0) {
+      qlin = 0.0;
+      qquad = 0.0;
+    } else {
+      double delvxxi = d->delv_xi[i] * d->delx_xi[i];
+      double delvxeta = d->delv_eta[i] * d->delx_eta[i];
+      double delvxzeta = d->delv_zeta[i] * d->delx_zeta[i];
+      if (delvxxi > 0.0) delvxxi = 0.0;
+      if (delvxeta > 0.0) delvxeta = 0.0;
+      if (delvxzeta > 0.0) delvxzeta = 0.0;
+      double rho = d->elemMass[i] / (d->volo[i] * d->vnew[i]);
+      qlin = ((-qlc) * rho) * (((delvxxi * (1.0 - phixi)) + (delvxeta * (1.0 - phieta))) +
+                               (delvxzeta * (1.0 - phizeta)));
+      qquad = (qqc * rho) * ((((delvxxi * delvxxi) * (1.0 - (phixi * phixi))) +
+                              ((delvxeta * delvxeta) * (1.0 - (phieta * phieta)))) +
+                             ((delvxzeta * delvxzeta) * (1.0 - (phizeta * phizeta))));
+    }
+    d->qq[i] = qquad;
+    d->ql[i] = qlin;
+  }
+  return err;
+}
+
+// src/domain.jl:2585-2614
+inline double pressure(double &bvc, double &pbvc, double e_old, double compression, double vnewc,
+                       double pmin, double p_cut, double eosvmax) {
+  const double c1s = 2.0 / 3.0;
+  bvc = c1s * (compression + 1.0);
+  pbvc = c1s;
+  double p_new = bvc * e_old;
+  if (std::fabs(p_new) < p_cut) p_new = 0.0;
+  if (vnewc >= eosvmax) p_new = 0.0;
+  if (p_new < pmin) p_new = pmin;
+  return p_new;
+}
+
+struct EosParams {
+  double e_cut, p_cut, q_cut, eosvmax, eosvmin, pmin, emin, rho0;
+};
+
+// one element of src/domain.jl:2752-2857 with 2617-2749 (work == 0)
+inline void eos_elem(const EosParams &P, double vnewc, double delvc, double e_old, double p_old,
+                     double q_old, double ql, double qq, double &p_out, double &e_out,
+                     double &q_out, double &ss_out) {
+  const double TINY1 = 0.111111e-36, TINY3 = 0.333333e-18, SIXTH = 1.0 / 6.0;
+  double compression = 1.0 / vnewc - 1.0;
+  double vchalf = vnewc - delvc * 0.5;
+  double compHalfStep = (1.0 / vchalf) - 1.0;
+  if (P.eosvmin != 0.0)
+    if (vnewc <= P.eosvmin) compHalfStep = compression;
+  if (P.eosvmax != 0.0)
+    if (vnewc >= P.eosvmax) {
+      p_old = 0.0;
+      compression = 0.0;
+      compHalfStep = 0.0;
+    }
+  const double work = 0.0;
+  double bvc, pbvc;
+  // calcEnergyForElems
+  double e_new = (e_old - ((0.5 * delvc) * (p_old + q_old))) + (0.5 * work);
+  if (e_new < P.emin) e_new = P.emin;
+  double pHalfStep = pressure(bvc, pbvc, e_new, compHalfStep, vnewc, P.pmin, P.p_cut, P.eosvmax);
+  double vhalf = 1.0 / (1.0 + compHalfStep);
+  double q_new;
+  if (delvc > 0.0) {
+    q_new = 0.0;
+  } else {
+    double ssc = ((pbvc * e_new) + (((vhalf * vhalf) * bvc) * pHalfStep)) / P.rho0;
+    if (ssc <= TINY1) ssc = TINY3;
+    else ssc = std::sqrt(ssc);
+    q_new = (ssc * ql) + qq;
+  }
+  e_new = e_new + ((0.5 * delvc) * ((3.0 * (p_old + q_old)) - (4.0 * (pHalfStep + q_new))));
+  e_new = e_new + 0.5 * work;
+  if (std::fabs(e_new) < P.e_cut) e_new = 0.0;
+  if (e_new < P.emin) e_new = P.emin;
+  double p_new = pressure(bvc, pbvc, e_new, compression, vnewc, P.pmin, P.p_cut, P.eosvmax);
+  double q_tilde;
+  if (delvc > 0.0) {
+    q_tilde = 0.0;
+  } else {
+    double ssc = ((pbvc * e_new) + (((vnewc * vnewc) * bvc) * p_new)) / P.rho0;
+    if (ssc <= TINY1) ssc = TINY3;
+    else ssc = std::sqrt(ssc);
+    q_tilde = (ssc * ql) + qq;
+  }
+  e_new = e_new - (((((7.0 * (p_old + q_old)) - (8.0 * (pHalfStep + q_new))) + (p_new + q_tilde)) *
+                    delvc) *
+                   SIXTH);
+  if (std::fabs(e_new) < P.e_cut) e_new = 0.0;
+  if (e_new < P.emin) e_new = P.emin;
+  p_new = pressure(bvc, pbvc, e_new, compression, vnewc, P.pmin, P.p_cut, P.eosvmax);
+  if (delvc <= 0.0) {
+    double ssc = ((pbvc * e_new) + (((vnewc * vnewc) * bvc) * p_new)) / P.rho0;
+    if (ssc <= TINY1) ssc = TINY3;
+    else ssc = std::sqrt(ssc);
+    q_new = (ssc * ql) + qq;
+    if (std::fabs(q_new) < P.q_cut) q_new = 0.0;
+  }
+  // calcSoundSpeedForElems, src/domain.jl:2732-2749
+  double ssTmp = ((pbvc * e_new) + (((vnewc * vnewc) * bvc) * p_new)) / P.rho0;
+  if (ssTmp <= TINY1) ssTmp = TINY3;
+  else ssTmp = std::sqrt(ssTmp);
+  p_out = p_new;
+  e_out = e_new;
+  q_out = q_new;
+  ss_out = ssTmp;
+}
+
+EosParams eos_params(const oracle_domain *d) {
+  EosParams P;
+  P.e_cut = d->e_cut; P.p_cut = d->p_cut; P.q_cut = d->q_cut;
+  P.eosvmax = d->eosvmax; P.eosvmin = d->eosvmin; P.pmin = d->pmin; P.emin = d->emin;
+  P.rho0 = d->refdens;
+  return P;
+}
+
+// src/domain.jl:2860-2910
+int material_properties(oracle_domain *d) {
+  const int64_t E = d->numElem;
+  if (E == 0) return 0;
+  const double eosvmin = d->eosvmin, eosvmax = d->eosvmax;
+  for (int64_t ie = 0; ie < E; ++ie) {
+    const int64_t zn = d->matElemlist[ie] - 1;
+    double vc = d->v[zn];
+    if (eosvmin != 0.0)
+      if (vc < eosvmin) vc = eosvmin;
+    if (eosvmax != 0.0)
+      if (vc > eosvmax) vc = eosvmax;
+    if (vc <= 0.0) return E_EOS_VOLUME;
+  }
+  const EosParams P = eos_params(d);
+  OMP_FOR
+  for (int64_t ie = 0; ie < E; ++ie) {
+    const int64_t zidx = d->matElemlist[ie] - 1;
+    double vnewc = d->vnew[zidx];
+    if (eosvmin != 0.0)
+      if (vnewc < eosvmin) vnewc = eosvmin;
+    if (eosvmax != 0.0)
+      if (vnewc > eosvmax) vnewc = eosvmax;
+    double pn, en, qn, ssn;
+    eos_elem(P, vnewc, d->delv[zidx], d->e[zidx], d->p[zidx], d->q[zidx], d->ql[zidx],
+             d->qq[zidx], pn, en, qn, ssn);
+    d->p[zidx] = pn;
+    d->e[zidx] = en;
+    d->q[zidx] = qn;
+    d->ss[zidx] = ssn;
+  }
+  return 0;
+}
+
+// src/domain.jl:2912-2930
+int update_volumes(oracle_domain *d) {
+  const int64_t E = d->numElem;
+  int err = 0;
+  for (int64_t i = 0; i < E; ++i) {
+    double tmpV = d->vnew[i];
+    if (std::fabs(tmpV - 1.0) < d->v_cut) tmpV = 1.0;
+    d->v[i] = tmpV;
+    if (tmpV <= 0.0) err = E_UPDATE_VOLUME;
+  }
+  return err;
+}
+
+}  // namespace
+
+// src/domain.jl:2932-2962 (with calcQForElems 2541-2583)
+extern "C" int oracle_lagrange_elements(oracle_domain *d) {
+  int err = lagrange_kinematics(d, d->deltatime);
+  if (err) return err;
+  monoq_gradients(d);
+  err = monoq_region(d);
+  if (err) return err;
+  for (int64_t i = 0; i < d->numElem; ++i)
+    if (d->q[i] > d->qstop) return E_QSTOP;
+  err = material_properties(d);
+  if (err) return err;
+  return update_volumes(d);
+}
+
+// src/domain.jl:2964-3064
+extern "C" int oracle_calc_time_constraints(oracle_domain *d) {
+  const int64_t E = d->numElem;
+  const double qqc2 = (64.0 * d->qqc) * d->qqc;
+  double dtc = 1.0e+20;
+  int64_t celem = -1;
+  for (int64_t ie = 0; ie < E; ++ie) {
+    const int64_t i = d->matElemlist[ie] - 1;
+    double dtf = d->ss[i] * d->ss[i];
+    if (d->vdov[i] < 0.0)
+      dtf = dtf + ((((qqc2 * d->arealg[i]) * d->arealg[i]) * d->vdov[i]) * d->vdov[i]);
+    dtf = std::sqrt(dtf);
+    dtf = d->arealg[i] / dtf;
+    if (d->vdov[i] != 0.0)
+      if (dtf < dtc) {
+        dtc = dtf;
+        celem = i + 1;
+      }
+  }
+  if (celem != -1) d->dtcourant = dtc;
+  double dth = 1.0e+20;
+  int64_t helem = -1;
+  for (int64_t ie = 0; ie < E; ++ie) {
+    const int64_t i = d->matElemlist[ie] - 1;
+    if (d->vdov[i] != 0.0) {
+      double dtdvov = d->dvovmax / (std::fabs(d->vdov[i]) + 1.e-20);
+      if (dth > dtdvov) {
+        dth = dtdvov;
+        helem = i + 1;
+      }
+    }
+  }
+  if (helem != -1) d->dthydro = dth;
+  return 0;
+}
+
+// src/domain.jl:3067-3080
+extern "C" int oracle_lagrange_leapfrog(oracle_domain *d) {
+  int err = oracle_lagrange_nodal(d);
+  if (err) return err;
+  err = oracle_lagrange_elements(d);
+  if (err) return err;
+  return oracle_calc_time_constraints(d);
+}
+
+// examples/benchmark.jl:80-101
+extern "C" int oracle_run(oracle_domain *d, int64_t num_iters, double *elapsed_s) {
+  auto t0 = std::chrono::steady_clock::now();
+  int err = 0;
+  while (d->time < d->stoptime) {
+    oracle_time_increment(d);
+    err = oracle_lagrange_leapfrog(d);
+    if (err) break;
+    if (d->cycle >= num_iters) break;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  if (elapsed_s) *elapsed_s = std::chrono::duration<double>(t1 - t0).count();
+  return err;
+}
+
+// src/utils.jl:124-152
+extern "C" void oracle_verify(const oracle_domain *d, double out[4]) {
+  const int64_t nx = d->sizeX;
+  double MaxAbsDiff = 0.0, MaxRelDiff = 0.0, TotalAbsDiff = 0.0;
+  for (int64_t j = 1; j <= nx; ++j)
+    for (int64_t k = j + 1; k <= nx; ++k) {
+      double a = d->e[(j - 1) * nx + k - 1], b = d->e[(k - 1) * nx + j - 1];
+      double AbsDiff = std::fabs(a - b);
+      TotalAbsDiff = TotalAbsDiff + AbsDiff;
+      if (MaxAbsDiff < AbsDiff) MaxAbsDiff = AbsDiff;
+      double RelDiff = AbsDiff / b;
+      if (MaxRelDiff < RelDiff) MaxRelDiff = RelDiff;
+    }
+  out[0] = d->e[0];
+  out[1] = MaxAbsDiff;
+  out[2] = TotalAbsDiff;
+  out[3] = MaxRelDiff;
+}
+
+extern "C" double *oracle_dfield(oracle_domain *d, const char *name, int64_t *n) {
+  std::string s(name);
+  std::vector<double> *f = nullptr;
+#define F(nm) else if (s == #nm) f = &d->nm;
+  if (false) {}
+  F(e) F(p) F(q) F(ql) F(qq) F(v) F(volo) F(delv) F(vdov) F(arealg) F(ss) F(elemMass)
+  F(vnew) F(delv_xi) F(delv_eta) F(delv_zeta) F(delx_xi) F(delx_eta) F(delx_zeta)
+  F(dxx) F(dyy) F(dzz) F(x) F(y) F(z) F(xd) F(yd) F(zd) F(xdd) F(ydd) F(zdd)
+  F(fx) F(fy) F(fz) F(nodalMass) F(determ)
+#undef F
+  if (!f) { if (n) *n = 0; return nullptr; }
+  if (n) *n = (int64_t)f->size();
+  return f->data();
+}
+
+extern "C" int64_t *oracle_ifield(oracle_domain *d, const char *name, int64_t *n) {
+  std::string s(name);
+  std::vector<int64_t> *f = nullptr;
+#define F(nm) else if (s == #nm) f = &d->nm;
+  if (false) {}
+  F(matElemlist) F(nodelist) F(lxim) F(lxip) F(letam) F(letap) F(lzetam) F(lzetap)
+  F(symmX) F(symmY) F(symmZ) F(nodeElemCount) F(nodeElemStart) F(nodeElemCornerList)
+#undef F
+  if (!f) { if (n) *n = 0; return nullptr; }
+  if (n) *n = (int64_t)f->size();
+  return f->data();
+}
+
+extern "C" uint32_t *oracle_elemBC(oracle_domain *d, int64_t *n) {
+  if (n) *n = (int64_t)d->elemBC.size();
+  return d->elemBC.data();
+}
+
+#define SCALARS(X)                                                                          \
+  X(dtfixed) X(deltatimemultlb) X(deltatimemultub) X(stoptime) X(dtmax) X(dthydro)          \
+  X(dtcourant) X(time) X(deltatime) X(u_cut) X(hgcoef) X(qstop) X(monoq_max_slope)           \
+  X(monoq_limiter_mult) X(e_cut) X(p_cut) X(ss4o3) X(q_cut) X(v_cut) X(qlc_monoq)           \
+  X(qqc_monoq) X(qqc) X(eosvmax) X(eosvmin) X(pmin) X(emin) X(dvovmax) X(refdens)
+
+extern "C" int oracle_get_scalar(const oracle_domain *d, const char *name, double *v) {
+  std::string s(name);
+  if (s == "cycle") { *v = (double)d->cycle; return 0; }
+  if (s == "numElem") { *v = (double)d->numElem; return 0; }
+  if (s == "numNode") { *v = (double)d->numNode; return 0; }
+#define X(nm) if (s == #nm) { *v = d->nm; return 0; }
+  SCALARS(X)
+#undef X
+  return -1;
+}
+
+extern "C" int oracle_set_scalar(oracle_domain *d, const char *name, double v) {
+  std::string s(name);
+  if (s == "cycle") { d->cycle = (int64_t)v; return 0; }
+#define X(nm) if (s == #nm) { d->nm = v; return 0; }
+  SCALARS(X)
+#undef X
+  return -1;
+}
+
+// ---- element-level exports for the pin tests ----
+extern "C" double oracle_cbrt(double x) { return cbrt_cr(x); }
+extern "C" double oracle_calc_elem_volume(const double *x, const double *y, const double *z) {
+  return elem_volume(x, y, z);
+}
+extern "C" void oracle_calc_elem_node_normals(const double *x, const double *y, const double *z,
+                                              int quirks, double *B) {
+  double b[8][3];
+  elem_node_normals(x, y, z, quirks, b);
+  std::memcpy(B, b, sizeof b);
+}
+extern "C" double oracle_calc_elem_shape_function_derivatives(const double *x, const double *y,
+                                                              const double *z, double *b) {
+  double bb[8][3];
+  double v = shape_derivatives(x, y, z, bb);
+  std::memcpy(b, bb, sizeof bb);
+  return v;
+}
+extern "C" void oracle_calc_elem_volume_derivative(const double *x, const double *y,
+                                                   const double *z, int quirks, double *dvdx,
+                                                   double *dvdy, double *dvdz) {
+  elem_volume_derivative(x, y, z, quirks, dvdx, dvdy, dvdz);
+}
+extern "C" double oracle_calc_elem_characteristic_length(const double *x, const double *y,
+                                                         const double *z, double volume) {
+  return char_length(x, y, z, volume);
+}
+extern "C" void oracle_calc_elem_velocity_gradient(const double *xv, const double *yv,
+                                                   const double *zv, const double *b, double detJ,
+                                                   double *d) {
+  double bb[8][3];
+  std::memcpy(bb, b, sizeof bb);
+  velocity_gradient(xv, yv, zv, bb, detJ, d);
+}
+extern "C" void oracle_calc_elem_fb_hourglass(const double *x, const double *y, const double *z,
+                                              const double *xd, const double *yd, const double *zd,
+                                              double determ, double ss, double mass, double hgcoef,
+                                              int quirks, double *hgf) {
+  double dvdx[8], dvdy[8], dvdz[8], fx[8], fy[8], fz[8];
+  elem_volume_derivative(x, y, z, quirks, dvdx, dvdy, dvdz);
+  fb_hourglass_elem(x, y, z, dvdx, dvdy, dvdz, determ, xd, yd, zd, ss, mass, hgcoef, fx, fy, fz);
+  for (int n = 0; n < 8; ++n) {
+    hgf[3 * n] = fx[n];
+    hgf[3 * n + 1] = fy[n];
+    hgf[3 * n + 2] = fz[n];
+  }
+}
+extern "C" void oracle_eval_eos_elem(double vnew, double v_unused, double delv, double e_old,
+                                     double p_old, double q_old, double ql, double qq,
+                                     double *out) {
+  (void)v_unused;
+  EosParams P;
+  P.e_cut = 1.0e-7; P.p_cut = 1.0e-7; P.q_cut = 1.0e-7;
+  P.eosvmax = 1.0e+9; P.eosvmin = 1.0e-9; P.pmin = 0.0; P.emin = -1.0e+15; P.rho0 = 1.0;
+  double vnewc = vnew;
+  if (vnewc < P.eosvmin) vnewc = P.eosvmin;
+  if (vnewc > P.eosvmax) vnewc = P.eosvmax;
+  eos_elem(P, vnewc, delv, e_old, p_old, q_old, ql, qq, out[0], out[1], out[2], out[3]);
+}

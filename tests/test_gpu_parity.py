@@ -1,0 +1,294 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bar: bit-identical FP64 state.  Both sides follow the reference's operation
+order with no FMA contraction, correctly rounded sqrt and division and the
+same correctly rounded cbrt, so every field must match to the last bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lulesh_amd import (Domain, LuleshError, LuleshProblem, calcTimeConstraintsForElems,
+                        lagrangeElements, lagrangeLeapFrog, lagrangeNodal, run, timeIncrement,
+                        verify)
+
+pytestmark = pytest.mark.gpu
+
+STATE_E = ("e", "p", "q", "v", "ss", "delv", "vdov", "arealg", "vnew", "volo", "elemMass")
+STATE_N = ("x", "y", "z", "xd", "yd", "zd", "nodalMass")
+TEMP_E = ("delx_xi", "delx_eta", "delx_zeta", "delv_xi", "delv_eta", "delv_zeta", "ql", "qq",
+          "dxx", "dyy", "dzz")
+AUX_N = ("fx", "fy", "fz", "xdd", "ydd", "zdd")
+SCAL = ("time", "deltatime", "cycle", "dtcourant", "dthydro")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def assert_same(dom, od, names, tag=""):
+    for n in names:
+        g, o = dom.get(n), od.field(n)
+        if not np.array_equal(bits(g), bits(o)):
+            bad = np.nonzero(bits(g) != bits(o))[0]
+            i = bad[0]
+            raise AssertionError(f"{tag} field {n}: {bad.size} mismatches, first at {i}: "
+                                 f"gpu={g[i]!r} oracle={o[i]!r}")
+
+
+def assert_scalars(dom, od, tag=""):
+    for n in SCAL:
+        g, o = dom.get(n), od.get(n)
+        assert bits([g])[0] == bits([o])[0], f"{tag} scalar {n}: gpu={g!r} oracle={o!r}"
+
+
+def make(nx, quirks=O.QUIRKS_REFERENCE, nthreads=1):
+    dom = Domain(LuleshProblem(10, True, nx, 11, 1, 1), quirks=quirks)
+    od = O.OracleDomain(nx, quirks, nthreads)
+    return dom, od
+
+
+@pytest.mark.parametrize("nx", [1, 2, 8, 12, 30])
+def test_initial_domain_bitwise(nx):
+    print(f"[init] nx={nx}", flush=True)
+    dom, od = make(nx)
+    assert_same(dom, od, STATE_E + STATE_N, f"init nx={nx}")
+    assert_scalars(dom, od, f"init nx={nx}")
+    assert np.array_equal(dom.get("elemBC"), od.elemBC())
+    dom.close()
+
+
+@pytest.mark.parametrize("quirks", [7, 0])
+def test_phase_by_phase_nx8(quirks):
+    print(f"[phases] nx=8 quirks={quirks}", flush=True)
+    dom, od = make(8, quirks)
+    dom.keep_aux(True)
+    for c in range(25):
+        timeIncrement(dom)
+        od.time_increment()
+        assert_scalars(dom, od, f"cycle {c + 1} dt")
+        lagrangeNodal(dom)
+        assert od.nodal() == 0
+        assert_same(dom, od, STATE_N + AUX_N, f"cycle {c + 1} nodal")
+        lagrangeElements(dom)
+        assert od.elements() == 0
+        assert_same(dom, od, STATE_E + TEMP_E, f"cycle {c + 1} elements")
+        calcTimeConstraintsForElems(dom)
+        assert od.constraints() == 0
+        assert_scalars(dom, od, f"cycle {c + 1} constraints")
+    dom.close()
+
+
+@pytest.mark.parametrize("nx,cycles", [(8, 300), (12, 150), (30, 60)])
+def test_leapfrog_cycles_bitwise(nx, cycles):
+    print(f"[cycles] nx={nx} cycles={cycles}", flush=True)
+    dom, od = make(nx, nthreads=4)
+    for _ in range(cycles):
+        timeIncrement(dom)
+        lagrangeLeapFrog(dom)
+        assert od.cycle_once() == 0
+    assert_same(dom, od, STATE_E + STATE_N, f"nx={nx} after {cycles}")
+    assert_scalars(dom, od, f"nx={nx} after {cycles}")
+    dom.close()
+
+
+@pytest.mark.parametrize("nx", [8, 12])
+def test_run_to_stoptime(nx):
+    """Driver loop to stoptime 1e-2 on the device vs the oracle's loop:
+    identical iteration count, final time and verify numbers."""
+    print(f"[run] nx={nx}", flush=True)
+    dom, od = make(nx, nthreads=4)
+    ncyc = run(dom, 10**9)
+    rc, _ = od.run(10**9)
+    assert rc == 0
+    assert ncyc == od.get("cycle") == dom.cycle
+    assert_scalars(dom, od, f"run nx={nx}")
+    g, o = verify(dom), od.verify()
+    assert bits(g).tolist() == bits(o).tolist(), (g, o)
+    assert_same(dom, od, STATE_E + STATE_N, f"run nx={nx}")
+    dom.close()
+
+
+def test_run_num_iters_and_graph_replay_matches_stepping():
+    print("[run-iters] nx=12", flush=True)
+    dom_a = Domain(LuleshProblem(10, True, 12, 11, 1, 1))
+    dom_b = Domain(LuleshProblem(10, True, 12, 11, 1, 1))
+    assert run(dom_a, 37) == 37
+    for _ in range(37):
+        timeIncrement(dom_b)
+        lagrangeLeapFrog(dom_b)
+    for n in STATE_E + STATE_N:
+        assert np.array_equal(bits(dom_a.get(n)), bits(dom_b.get(n))), n
+    # a second run continues from where the first stopped
+    assert run(dom_a, 50) == 13
+    assert dom_a.cycle == 50
+    dom_a.close()
+    dom_b.close()
+
+
+def test_quirks_off_nx30_full_run_matches_lulesh2():
+    """Quirks cleared = LULESH 2.0 arithmetic: the published sample output
+    for -s 30 is 932 iterations and Final Origin Energy 2.025075e+05."""
+    print("[lulesh2] nx=30 quirks=0", flush=True)
+    dom, od = make(30, quirks=0, nthreads=8)
+    ncyc = run(dom, 10**9)
+    assert ncyc == 932
+    assert "%.6e" % verify(dom)[0] == "2.025075e+05"
+    rc, _ = od.run(10**9)
+    assert rc == 0
+    g, o = verify(dom), od.verify()
+    assert bits(g).tolist() == bits(o).tolist(), (g, o)
+    assert_scalars(dom, od, "lulesh2 nx=30")
+    dom.close()
+
+
+def test_nx192_first_cycles_bitwise():
+    """The headline shape itself (s=192): three cycles, GPU vs oracle."""
+    print("[nx192] 3 cycles", flush=True)
+    nthr = min(16, os.cpu_count() or 1)
+    dom, od = make(192, nthreads=nthr)
+    for _ in range(3):
+        timeIncrement(dom)
+        lagrangeLeapFrog(dom)
+        assert od.cycle_once() == 0
+    assert_same(dom, od, ("e", "p", "q", "v", "ss", "x", "xd", "y", "z"), "nx=192")
+    assert_scalars(dom, od, "nx=192")
+    dom.close()
+
+
+def test_deterministic_reruns_nx96():
+    print("[determinism] nx=96", flush=True)
+    outs = []
+    for _ in range(2):
+        dom = Domain(LuleshProblem(10, True, 96, 11, 1, 1))
+        run(dom, 40)
+        outs.append({n: dom.get(n) for n in ("e", "p", "x", "xd", "ss")})
+        dom.close()
+    for n in outs[0]:
+        assert np.array_equal(bits(outs[0][n]), bits(outs[1][n])), n
+
+
+# ---- errors: each reference error() message, forced from the host ----
+
+def _err_case(prep_gpu, prep_oracle):
+    dom, od = make(8)
+    for _ in range(3):
+        timeIncrement(dom)
+        lagrangeLeapFrog(dom)
+        od.cycle_once()
+    prep_gpu(dom)
+    prep_oracle(od)
+    timeIncrement(dom)
+    od.time_increment()
+    with pytest.raises(LuleshError) as ei:
+        lagrangeLeapFrog(dom)
+    rc = od.leapfrog()
+    dom.close()
+    return ei.value, rc
+
+
+def test_error_negative_volume():
+    def g(d):
+        v = d.v
+        v[5] = -1.0
+        d.v = v
+
+    def o(od):
+        v = od.field("v")
+        v[5] = -1.0
+        od.set_field("v", v)
+    exc, rc = _err_case(g, o)
+    assert str(exc) == "Volume Error: Volume is negative" and exc.code == rc == 3
+
+
+def test_error_qstop():
+    exc, rc = _err_case(lambda d: setattr(d, "qstop", -1.0), lambda od: od.set("qstop", -1.0))
+    assert str(exc) == "QStopError" and exc.code == rc == 7
+
+
+def test_error_invalid_bc():
+    def g(d):
+        bc = d.elemBC
+        bc[40] |= 0x1 | 0x2  # XI_M_SYMM | XI_M_FREE
+        d.elemBC = bc
+
+    def o(od):
+        bc = od.elemBC()
+        bc[40] |= 0x1 | 0x2
+        od.set_elemBC(bc)
+    exc, rc = _err_case(g, o)
+    assert str(exc) == "Error" and exc.code == rc == 6
+
+
+def test_error_inverted_element():
+    def flip(x):
+        x = x.copy()
+        x[0], x[1] = 0.5, 0.0  # node 1 moves past node 2 along x
+        return x
+
+    exc, rc = _err_case(lambda d: setattr(d, "x", flip(d.x)), lambda od: od.set_field("x", flip(od.field("x"))))
+    assert str(exc) == "Early Volume Error" and exc.code == rc == 1
+
+
+# ---- device arithmetic ----
+
+def _exact_cbrt_cr(x):
+    """correctly rounded cbrt of a positive double by exact integer search"""
+    from fractions import Fraction
+    m, e = np.frexp(x)
+    M = int(np.ldexp(m, 53))
+    E = int(e) - 53
+    q, r = divmod(E, 3)
+    X = M << r  # x = X * 2^(3q)
+    # y = cbrt(X) * 2^q; find K = round(cbrt(X) * 2^k) with 53 significant bits
+    k = 60
+    T = X << (3 * k)
+    lo, hi = 0, 1 << 200
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        if mid ** 3 <= T:
+            lo = mid
+        else:
+            hi = mid
+    root = lo  # floor(cbrt(X) * 2^k)
+    nbits = root.bit_length()
+    shift = nbits - 53
+    K, rem = root >> shift, root & ((1 << shift) - 1)
+    # decide rounding exactly against the midpoint (2K+1)/2 * 2^shift
+    mid = (2 * K + 1) << (shift - 1)
+    if Fraction(mid) ** 3 < T:
+        K += 1
+    return float(np.ldexp(float(K), shift - k + q))
+
+
+def test_device_cbrt_is_correctly_rounded():
+    import ctypes
+    from lulesh_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(1e-6, 10.0, 4000), np.exp(rng.uniform(-700, 700, 2000)),
+                         [1.0, 8.0, 27.0, 1.125 ** 3 / 27000, 5e-324, 2.2e-308, 1.7e308]])
+    out = np.zeros_like(xs)
+    P = _lib.P_double
+    assert L.lulesh_math_check(0, xs.ctypes.data_as(P), xs.ctypes.data_as(P),
+                               out.ctypes.data_as(P), xs.size, 0) == 0
+    ref = np.array([O.cbrt(v) for v in xs])
+    assert np.array_equal(bits(out), bits(ref))
+    for v, g in zip(xs[::97], out[::97]):
+        assert g == _exact_cbrt_cr(float(v)), v
+
+
+def test_device_sqrt_and_division_are_ieee():
+    from lulesh_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(1)
+    a = np.exp(rng.uniform(-300, 300, 20000))
+    b = np.exp(rng.uniform(-300, 300, 20000)) * rng.choice([-1.0, 1.0], 20000)
+    out = np.zeros_like(a)
+    P = _lib.P_double
+    assert L.lulesh_math_check(1, a.ctypes.data_as(P), b.ctypes.data_as(P), out.ctypes.data_as(P), a.size, 0) == 0
+    assert np.array_equal(bits(out), bits(np.sqrt(a)))
+    assert L.lulesh_math_check(2, a.ctypes.data_as(P), b.ctypes.data_as(P), out.ctypes.data_as(P), a.size, 0) == 0
+    assert np.array_equal(bits(out), bits(a / b))
