@@ -1,0 +1,148 @@
+"""Throughput of the LULESH Lagrange leap-frog cycle on one MI355X.
+
+Workload: the Sedov blast at s=192 (7,077,888 elements, 7,189,057 nodes),
+FP64, the first `--steps` cycles (default 500) from the initial state.  A
+step is one cycle of the reference's driver loop: timeIncrement! +
+lagrangeLeapFrog.  The domain is built on the device and stays resident in
+HBM; warm-up cycles run on the same context, which is then reset to the
+initial state before the timed region.
+
+Prints ONE JSON line: zone-cycles/s, the roofline of the dominant kernel
+(algorithmic bytes per launch / mean launch time from HIP events recorded on
+the context's stream over the timed region) and the CPU baseline (the
+oracle, one thread, on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "lulesh.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def alg_bytes(E, N):
+    """Algorithmic HBM bytes per launch of each kernel (unique footprint:
+    every array element a kernel needs counted once, 8 B per double, 4 B per
+    elemBC mask; nodal gathers count each node once)."""
+    return {
+        # x,y,z,xd,yd,zd (48N) + p,q,v,volo,ss,elemMass (48E) + 6 corner arrays written (384E)
+        "k_force": 432 * E + 48 * N,
+        # 6 corner arrays read (384E) + nodalMass (8N) + x..zd read and written (96N)
+        "k_node": 384 * E + 104 * N,
+        # x..zd (48N) + volo, v (16E) + vnew delv arealg vdov delx*3 delv*3 written (80E)
+        "k_kin": 96 * E + 48 * N,
+        # elemBC 4 + delv*3 24 + delx*3 24 + vdov elemMass volo vnew v delv e p q arealg 80
+        # read, e p q ss v 40 written
+        "k_qeos": 172 * E,
+        "k_dt": 16 * ((E + 255) // 256),
+    }
+
+
+def cpu_baseline(nx, cycles, nthreads):
+    import oracle as O
+    od = O.OracleDomain(nx, O.QUIRKS_REFERENCE, nthreads)
+    E = od.get("numElem")
+    t0 = time.perf_counter()
+    for _ in range(cycles):
+        rc = od.cycle_once()
+        if rc:
+            raise RuntimeError(f"oracle error {rc}")
+    dt = time.perf_counter() - t0
+    od.close()
+    return E * cycles / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=192)
+    ap.add_argument("--cpu-cycles", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs one single-GPU domain; multi-GPU (--gpus > 1) is not "
+              "part of it (see DESIGN.md for how the domain would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    from lulesh_amd import Domain, LuleshProblem, run
+
+    nx = args.nx
+    dom = Domain(LuleshProblem(args.steps, True, nx, 11, 1, 1))
+    E, N = dom.numElem, dom.numNode
+    if args.warmup > 0:
+        run(dom, args.warmup)
+    dom.reset()
+    dom.set_timing(True)
+    t0 = time.perf_counter()
+    ncyc = run(dom, args.steps)
+    wall = time.perf_counter() - t0
+    kt = dom.kernel_times()
+    dom.set_timing(False)
+    if ncyc != args.steps:
+        raise RuntimeError(f"ran {ncyc} cycles, expected {args.steps}")
+    value = E * ncyc / wall
+
+    ab = alg_bytes(E, N)
+    kernels = {}
+    for name, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        avg = ms / cnt
+        gbs = ab[name] / (avg * 1e-3) / 1e9
+        kernels[name] = {"avg_ms": round(avg, 5), "launches": cnt,
+                         "alg_bytes": ab[name], "GB_s": round(gbs, 1)}
+    dominant = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
+    dk = kernels[dominant]
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as fh:
+                traffic = json.load(fh).get(dominant)
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": dk["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dk["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": dominant}
+
+    cpu = None
+    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        v, secs = cpu_baseline(nx, args.cpu_cycles, args.cpu_threads)
+        cpu = {"value": round(v, 1), "unit": "zone-cycles/s", "cores": args.cpu_threads,
+               "kind": "port",
+               "sample": f"CPU oracle (C++ restatement), s={nx}, cycles 1-{args.cpu_cycles} of the "
+                         f"same workload, {secs:.1f} s, {args.cpu_threads} thread(s)"}
+
+    out = {
+        "metric": "FP64 zone-cycles/s at s=192",
+        "value": round(value, 1),
+        "unit": "zone-cycles/s",
+        "n_gpus": 1,
+        "steps": ncyc,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / ncyc * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: Sedov blast initial state built on the device",
+        "config": {"workload": f"Sedov s={nx} first {ncyc} cycles", "nx": nx, "elements": E,
+                   "nodes": N, "cycles": ncyc, "quirks": "reference"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "alg_bytes_per_cycle": sum(ab.values()),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

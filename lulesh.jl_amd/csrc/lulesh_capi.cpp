@@ -469,8 +469,13 @@ int lulesh_run(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
     rc = build_graph(c);
     if (rc) return rc;
   }
-  int64_t batch = 8;
+  // replay exactly the cycles num_iters still allows (the first one always
+  // runs); only a stop-time exit leaves gated no-op replays behind
+  int64_t cap = 8;
+  long long next_cycle = cycle0 + 1;
   for (;;) {
+    const long long left = (long long)num_iters - next_cycle + 1;
+    const int64_t batch = left < 1 ? 1 : (left < cap ? left : cap);
     for (int64_t i = 0; i < batch; ++i) {
       if (use_graph) {
         HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
@@ -486,7 +491,8 @@ int lulesh_run(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
     rc = finish(c, false);
     if (rc) return rc;
     if (!c->hsc->active) break;
-    if (batch < 256) batch *= 2;
+    next_cycle = c->hsc->cycle;
+    if (cap < 256) cap *= 2;
   }
   c->partials_valid = false;
   if (cycles_run) *cycles_run = (int64_t)(c->hsc->cycle - cycle0);

@@ -1,0 +1,51 @@
+"""Writes tests/golden/oracle_runs.json: results of the CPU oracle on the
+parity configurations (verify numbers, cycle count, final time and SHA-256
+of the final energy and x fields).  Run from the repo root after
+`make -C oracle`:  python tests/golden/make_golden.py"""
+import hashlib
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+import oracle as O  # noqa: E402
+
+CASES = [  # (name, nx, quirks, num_iters)
+    ("s8_to_stoptime", 8, 7, 10 ** 9),
+    ("s12_to_stoptime", 12, 7, 10 ** 9),
+    ("s16_to_stoptime", 16, 7, 10 ** 9),
+    ("s8_smoke_10", 8, 7, 10),
+    ("s30_first_200", 30, 7, 200),
+    ("s30_quirk_free_to_stoptime", 30, 0, 10 ** 9),
+]
+
+
+def digest(a):
+    return hashlib.sha256(a.tobytes()).hexdigest()
+
+
+def run_case(nx, quirks, num_iters, nthreads=8):
+    od = O.OracleDomain(nx, quirks, nthreads)
+    rc, _ = od.run(num_iters)
+    v = od.verify()
+    return {"rc": rc, "cycle": od.get("cycle"), "time": od.get("time").hex(),
+            "deltatime": od.get("deltatime").hex(),
+            "verify": [float(x).hex() for x in v],
+            "verify_printed": ["%.6e" % x for x in v],
+            "sha256_e": digest(od.field("e")), "sha256_x": digest(od.field("x"))}
+
+
+def main():
+    out = {}
+    for name, nx, q, it in CASES:
+        r = run_case(nx, q, it)
+        r.update({"nx": nx, "quirks": q, "num_iters": it})
+        out[name] = r
+        print(name, r["cycle"], r["verify_printed"])
+    with open(os.path.join(HERE, "oracle_runs.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,62 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry
+point include/lulesh_hip.h declares, validates arguments before touching the
+device, and carries the reference's error texts."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from lulesh_amd import _lib
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "lulesh_hip.h")
+
+REFERENCE_TEXTS = {  # src/domain.jl error() messages, in program order
+    1: "Early Volume Error", 2: "Mid Volume Error", 3: "Volume Error: Volume is negative",
+    4: "Volume Error: calcKinematicsForElems", 5: "Volume Error :2157", 6: "Error",
+    7: "QStopError", 8: "Volume Error :2887", 9: "Volume Error :2908",
+}
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lulesh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    names = declared()
+    assert len(names) >= 20
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_error_texts_are_the_reference_messages():
+    lib = _lib.load()
+    for code, text in REFERENCE_TEXTS.items():
+        assert lib.lulesh_error_string(code).decode() == text
+    assert lib.lulesh_error_string(0).decode() == "ok"
+
+
+def test_argument_validation_before_device_use():
+    lib = _lib.load()
+    ctx = ctypes.c_void_p()
+    assert lib.lulesh_create(0, 11, 1, 1, 7, 0, ctypes.byref(ctx)) == 100
+    assert lib.lulesh_create(8, 11, 1, 1, 9, 0, ctypes.byref(ctx)) == 100
+    assert not ctx.value
+    assert lib.lulesh_math_check(7, None, None, None, 1, 0) == 100
+    assert lib.lulesh_destroy(None) == 100
+    assert lib.lulesh_kernel_name(0).decode() == "k_force"
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/liblulesh_hip.so")
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(ImportError):
+        _lib.load()

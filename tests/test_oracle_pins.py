@@ -1,0 +1,430 @@
+"""Pins of the CPU oracle against facts the reference's code does not hand us
+as data: the reference has no golden vectors or numeric tests (its tests are
+JET static checks and an exit-code smoke run, test/runtests.jl:25-50), so the
+oracle is pinned by closed forms, finite differences, invariants, brute
+force on tiny inputs and one external result (LULESH 2.0's published s=30
+output for the quirk-free arithmetic).  Each check is chosen so that a
+plausible slip (a dropped term, a wrong sign or index, a swapped operand)
+fails it.
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+UNIT = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                 [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=float)
+GAMMA = np.array([[1, 1, 1, -1], [1, -1, -1, 1], [-1, -1, 1, -1], [-1, 1, -1, 1],
+                  [-1, -1, 1, 1], [-1, 1, -1, -1], [1, 1, 1, 1], [1, -1, -1, -1]], float)
+
+
+def xyz(P):
+    return P[:, 0].copy(), P[:, 1].copy(), P[:, 2].copy()
+
+
+def parallelepiped(rng):
+    A = np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+    if np.linalg.det(A) < 0:
+        A[:, 0] *= -1
+    return UNIT @ A.T + rng.standard_normal(3), A
+
+
+def distorted(rng, amp=0.15):
+    return UNIT + amp * rng.standard_normal(UNIT.shape)
+
+
+def fd_volume_grad(P, h=1e-6):
+    x, y, z = xyz(P)
+    G = np.zeros((8, 3))
+    for n in range(8):
+        for c, a in enumerate((x, y, z)):
+            a0 = a[n]
+            a[n] = a0 + h
+            vp = O.elem_volume(x, y, z)
+            a[n] = a0 - h
+            vm = O.elem_volume(x, y, z)
+            a[n] = a0
+            G[n, c] = (vp - vm) / (2 * h)
+    return G
+
+
+def gauss_volume(P):
+    """trilinear hex volume by 2x2x2 Gauss quadrature of det J (exact: det J
+    of a trilinear map has degree <= 2 per reference coordinate)"""
+    g = np.array([-1, 1]) / np.sqrt(3.0)
+    xi = (UNIT * 2 - 1)  # nodes at +-1
+    vol = 0.0
+    for a in g:
+        for b in g:
+            for c in g:
+                p = np.array([a, b, c])
+                dN = np.zeros((8, 3))
+                for n in range(8):
+                    s = xi[n]
+                    f = (1 + s * p) / 2
+                    dN[n] = [s[0] / 2 * f[1] * f[2], s[1] / 2 * f[0] * f[2], s[2] / 2 * f[0] * f[1]]
+                J = P.T @ dN
+                vol += np.linalg.det(J)
+    return vol
+
+
+# ---------------- cube root ----------------
+
+def exact_cbrt(x):
+    m, e = np.frexp(x)
+    M, E = int(np.ldexp(m, 53)), int(e) - 53
+    q, r = divmod(E, 3)
+    X = M << r
+    k = 60
+    T = X << (3 * k)
+    lo, hi = 0, 1 << 200
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        lo, hi = (mid, hi) if mid ** 3 <= T else (lo, mid)
+    shift = lo.bit_length() - 53
+    K = lo >> shift
+    if Fraction((2 * K + 1) << (shift - 1)) ** 3 < T:
+        K += 1
+    return float(np.ldexp(float(K), shift - k + q))
+
+
+def test_cbrt_correctly_rounded():
+    rng = np.random.default_rng(0)
+    xs = list(rng.uniform(1e-3, 10, 300)) + list(np.exp(rng.uniform(-700, 700, 300)))
+    xs += [1.0, 8.0, 27.0, 1000.0, 2.0 ** -1074, 2.0 ** -1022, 1.7976931348623157e308,
+           (1.125 / 30) ** 3, (1.125 / 192) ** 3]
+    for x in xs:
+        assert O.cbrt(x) == exact_cbrt(float(x)), x
+        assert O.cbrt(-x) == -exact_cbrt(float(x))
+    assert O.cbrt(0.0) == 0.0 and np.isnan(O.cbrt(np.nan))
+
+
+# ---------------- element geometry ----------------
+
+def test_volume_unit_cube_and_parallelepipeds():
+    assert O.elem_volume(*xyz(UNIT)) == 1.0
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        P, A = parallelepiped(rng)
+        assert O.elem_volume(*xyz(P)) == pytest.approx(np.linalg.det(A), rel=1e-13)
+
+
+def test_volume_general_hex_matches_quadrature():
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        P = distorted(rng)
+        assert O.elem_volume(*xyz(P)) == pytest.approx(gauss_volume(P), rel=1e-12)
+
+
+def test_volume_derivative_slots_and_quirk_q2():
+    """Slots 1,3,5-8 hold dV/d(node); the reference's slot 2 holds node 4's
+    derivative and slot 4 node 2's (src/domain.jl:1216-1229)."""
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        P = distorted(rng)
+        G = fd_volume_grad(P)
+        d0 = np.stack(O.volume_derivative(*xyz(P), quirks=0), 1)
+        assert np.allclose(d0, G, atol=1e-8)
+        dq = np.stack(O.volume_derivative(*xyz(P), quirks=O.Q2), 1)
+        perm = [0, 3, 2, 1, 4, 5, 6, 7]
+        assert np.allclose(dq, G[perm], atol=1e-8)
+        assert not np.allclose(dq, G, atol=1e-4)
+
+
+def test_shape_derivatives_consistency():
+    """sum_n b[n,i] x_n[j] = detJ delta_ij for any hex (gradient of the
+    interpolated position at the centre is the identity); detJ equals the
+    exact volume for parallelepipeds; b rows 5..8 negate rows 3,4,1,2."""
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        P = distorted(rng)
+        b, dj = O.shape_derivatives(*xyz(P))
+        M = b.T @ P
+        assert np.allclose(M, dj * np.eye(3), atol=1e-13)
+        assert np.array_equal(b[4], -b[2]) and np.array_equal(b[5], -b[3])
+        assert np.array_equal(b[6], -b[0]) and np.array_equal(b[7], -b[1])
+        P2, A = parallelepiped(rng)
+        assert O.shape_derivatives(*xyz(P2))[1] == pytest.approx(np.linalg.det(A), rel=1e-13)
+    b, dj = O.shape_derivatives(*xyz(UNIT))
+    assert dj == 1.0
+
+
+def test_node_normals_and_quirk_q1():
+    """Quirk-free node normals sum to zero and, on parallelepipeds (planar
+    faces), equal dV/dx_n.  With Q1 node 1 receives face 5's areaY twice and
+    no areaZ (src/domain.jl:1047-1055), so the sum becomes (0, +a5y, -a5z)
+    where a5 is face 5's quarter area vector, (1/8) d13 x d02 over the face
+    nodes (4,8,5,1) -- a diagonal form independent of the bisector formula
+    the code uses."""
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        P = distorted(rng)
+        x, y, z = xyz(P)
+        B0 = O.node_normals(x, y, z, 0)
+        B1 = O.node_normals(x, y, z, O.Q1)
+        assert np.allclose(B0.sum(0), 0, atol=1e-14)
+        f0, f1, f2, f3 = P[3], P[7], P[4], P[0]
+        a5 = 0.125 * np.cross(f3 - f1, f2 - f0)
+        assert np.allclose(B1.sum(0), [0.0, a5[1], -a5[2]], atol=1e-14)
+        assert B1[0, 1] - B0[0, 1] == pytest.approx(a5[1], abs=1e-14)
+        assert B0[0, 2] - B1[0, 2] == pytest.approx(a5[2], abs=1e-14)
+        assert np.array_equal(B1[1:], B0[1:]) and B1[0, 0] == B0[0, 0]
+        P2, _ = parallelepiped(rng)
+        assert np.allclose(O.node_normals(*xyz(P2), 0), fd_volume_grad(P2), atol=1e-8)
+
+
+def test_characteristic_length():
+    for h in (1.0, 0.5, 1.125 / 30):
+        P = UNIT * h
+        assert O.char_length(*xyz(P), h ** 3) == pytest.approx(h, rel=1e-15)
+    # box with edges a<b<c: the largest face is b*c -> 4abc / (4 bc) = a
+    P = UNIT * np.array([0.2, 0.5, 0.7])
+    assert O.char_length(*xyz(P), 0.2 * 0.5 * 0.7) == pytest.approx(0.2, rel=1e-14)
+
+
+def test_velocity_gradient_exact_for_linear_fields():
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        P = distorted(rng)
+        b, dj = O.shape_derivatives(*xyz(P))
+        L = rng.standard_normal((3, 3))
+        V = P @ L.T + rng.standard_normal(3)
+        d = O.velocity_gradient(V[:, 0], V[:, 1], V[:, 2], b, dj)
+        assert np.allclose(d[:3], np.diag(L), atol=1e-12)
+        S = 0.5 * (L + L.T)
+        assert np.allclose(d[3:], [S[2, 1], S[2, 0], S[1, 0]], atol=1e-12)
+
+
+# ---------------- hourglass ----------------
+
+def test_hourglass_vanishes_for_affine_motion_on_parallelepipeds():
+    rng = np.random.default_rng(8)
+    for _ in range(20):
+        P, A = parallelepiped(rng)
+        V = P @ rng.standard_normal((3, 3)).T + rng.standard_normal(3)
+        det = np.linalg.det(A)
+        for q in (0, O.QUIRKS_REFERENCE):
+            f = O.fb_hourglass(*xyz(P), V[:, 0], V[:, 1], V[:, 2], det, 1.3, det, 3.0, q)
+            assert np.abs(f).max() < 1e-12
+
+
+@pytest.mark.parametrize("m", range(4))
+def test_hourglass_mode_on_cube_closed_form(m):
+    """On a cube of edge h (hourmod = Gamma^T x = 0, so hourgam = Gamma), a
+    velocity v_x = Gamma[:,m] gives hgf_x = -8*0.01*hgcoef*ss*mass/h * Gamma[:,m]."""
+    h, ss, mass, hg = 0.7, 2.0, 0.9, 3.0
+    P = UNIT * h
+    vx = GAMMA[:, m]
+    f = O.fb_hourglass(*xyz(P), vx, np.zeros(8), np.zeros(8), h ** 3, ss, mass, hg)
+    expect = -8 * 0.01 * hg * ss * mass / h * GAMMA[:, m]
+    assert np.allclose(f[:, 0], expect, rtol=1e-14, atol=1e-15)
+    assert np.abs(f[:, 1:]).max() == 0.0
+
+
+def test_hourglass_quirk_q2_acts_only_off_parallelepipeds():
+    rng = np.random.default_rng(9)
+    P = distorted(rng, 0.2)
+    V = rng.standard_normal((8, 3))
+    dv = O.elem_volume(*xyz(P))
+    f0 = O.fb_hourglass(*xyz(P), V[:, 0], V[:, 1], V[:, 2], dv, 1.0, dv, 3.0, 0)
+    f2 = O.fb_hourglass(*xyz(P), V[:, 0], V[:, 1], V[:, 2], dv, 1.0, dv, 3.0, O.Q2)
+    assert np.abs(f0 - f2).max() > 1e-6
+    # momentum: quirk-free hourglass forces sum to zero (Gamma-based modes)
+    assert np.allclose(f0.sum(0), 0, atol=1e-12)
+
+
+# ---------------- EOS ----------------
+
+def test_eos_at_rest_closed_form():
+    for e0 in (1.0, 1.169998815e7, 3.0e9):
+        p, e, q, ss = O.eos_elem(1.0, 0.0, e0, 0.0, 0.0, 0.0, 0.0)
+        assert e == e0 and p == pytest.approx(2.0 / 3.0 * e0, rel=1e-15)
+        assert ss == pytest.approx(np.sqrt(10.0 / 9.0 * e0), rel=1e-15) and q == 0.0
+    # cold element: sound speed floor TINY3 (src/domain.jl:2742-2744)
+    p, e, q, ss = O.eos_elem(1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    assert (p, e, q, ss) == (0.0, 0.0, 0.0, 0.333333e-18)
+
+
+def test_eos_compression_energy_balance():
+    """The three-stage energy update is Simpson's rule for pdV work:
+    e_new - e_old = -(P0 + 4 Phalf + Pnew)/6 * delv exactly in exact
+    arithmetic; for a state at rest (p0 = 2/3 e0) and small delv all three
+    pressures agree to O(delv), so de = -p0 delv (1 + O(delv)).  Also
+    p = (2/3) e / v, ss^2 = pbvc e + v^2 bvc p and q = ss*ql + qq."""
+    rng = np.random.default_rng(10)
+    for _ in range(20):
+        e0 = rng.uniform(1, 10)
+        p0 = 2.0 / 3.0 * e0
+        dv = -1e-5 * rng.uniform(0.5, 2)
+        vnew = 1.0 + dv
+        p, e, q, ss = O.eos_elem(vnew, dv, e0, p0, 0.0, 0.0, 0.0)
+        assert e - e0 == pytest.approx(-p0 * dv, rel=1e-4)
+        assert e > e0  # compression heats
+        assert p == pytest.approx(2.0 / 3.0 * e / vnew, rel=1e-13)
+        c2 = (2.0 / 3.0) * e + vnew * vnew * ((2.0 / 3.0) / vnew) * p
+        assert ss == pytest.approx(np.sqrt(c2), rel=1e-13)
+        ql, qq = rng.uniform(0, 1e-3), rng.uniform(0, 1e-3)
+        p, e, q, ss = O.eos_elem(vnew, dv, e0, p0, 0.0, ql, qq)
+        assert q == pytest.approx(ss * ql + qq, rel=1e-13)
+    # expansion (delv > 0) switches q off and cools
+    p, e, q, ss = O.eos_elem(1.0 + 1e-5, 1e-5, 5.0, 10.0 / 3.0, 0.0, 1.0, 1.0)
+    assert q == 0.0 and e < 5.0
+
+
+# ---------------- timeIncrement! branch table ----------------
+
+def test_time_increment_branches():
+    ti = O.time_increment_scalars
+    # cycle 0: dt untouched
+    assert ti(0.0, 1e-6, 0, 1e20, 1e20) == (1e-6, 1e-6, 1)
+    # gnewdt = dtcourant/2 = 1.05e-6 -> ratio 1.05 in [1, 1.1): keep olddt
+    t, dt, c = ti(0.0, 1e-6, 5, 2.1e-6, 1e20)
+    assert dt == 1e-6 and c == 6
+    # ratio 1.15 in [1.1, 1.2]: accept newdt
+    assert ti(0.0, 1e-6, 5, 2.3e-6, 1e20)[1] == 2.3e-6 / 2.0
+    # ratio > 1.2: capped at 1.2 olddt
+    assert ti(0.0, 1e-6, 5, 5e-6, 1e20)[1] == 1e-6 * 1.2
+    # ratio < 1: take newdt (hydro branch, dthydro*2/3 < dtcourant/2)
+    assert ti(0.0, 1e-6, 5, 1e20, 1.2e-6)[1] == (1.2e-6 * 2.0) / 3.0
+    # dtmax cap
+    assert ti(0.0, 1e-2, 5, 1.0, 1.0, dtmax=1e-2)[1] == 1e-2
+    # stop-time fit: dt < target < 4/3 dt -> 2/3 dt
+    t, dt, c = ti(1e-2 - 1.2e-6, 1e-6, 0, 1e20, 1e20)
+    assert dt == (2.0 * 1e-6) / 3.0
+    # target < dt -> land exactly on target
+    t, dt, c = ti(1e-2 - 0.5e-6, 1e-6, 0, 1e20, 1e20)
+    assert dt == 1e-2 - (1e-2 - 0.5e-6) and t == (1e-2 - 0.5e-6) + dt
+
+
+# ---------------- whole domain ----------------
+
+@pytest.mark.parametrize("nx", [1, 4, 8])
+def test_domain_construction(nx):
+    """Domain(prob) arrays against independent constructions of
+    src/domain.jl:117-174, 615-661, 773-916."""
+    od = O.OracleDomain(nx)
+    s, s1 = nx, nx + 1
+    E, N = s ** 3, s1 ** 3
+    n = np.arange(N)
+    col, row, plane = n % s1, (n // s1) % s1, n // (s1 * s1)
+    assert np.array_equal(od.field("x"), 1.125 * col / s)
+    assert np.array_equal(od.field("y"), 1.125 * row / s)
+    assert np.array_equal(od.field("z"), 1.125 * plane / s)
+    e = np.arange(E)
+    ec, er, ep = e % s, (e // s) % s, e // (s * s)
+    n0 = ep * s1 * s1 + er * s1 + ec
+    P = s1 * s1
+    nl = np.stack([n0, n0 + 1, n0 + s1 + 1, n0 + s1, n0 + P, n0 + P + 1, n0 + P + s1 + 1,
+                   n0 + P + s1], 1).ravel()
+    assert np.array_equal(od.ifield("nodelist"), nl)
+    # corner list: for every node, its corners in ascending element order, 1-based
+    cl = od.ifield("nodeElemCornerList")
+    start, count = od.ifield("nodeElemStart"), od.ifield("nodeElemCount")
+    for g in range(N):
+        cs = cl[start[g]:start[g] + count[g]]
+        assert np.all(np.diff(cs) > 0)
+        assert np.all(nl[cs - 1] == g)
+    assert count.sum() == 8 * E
+    # symmetry sets: col 0, row 0, plane 0 nodes
+    assert set(od.ifield("symmX")) == set(n[col == 0])
+    assert set(od.ifield("symmY")) == set(n[row == 0])
+    assert set(od.ifield("symmZ")) == set(n[plane == 0])
+    bc = od.elemBC()
+    expect = ((ep == 0) * 0x01000 | (ep == s - 1) * 0x10000 | (er == 0) * 0x00040 |
+              (er == s - 1) * 0x00400 | (ec == 0) * 0x00001 | (ec == s - 1) * 0x00010)
+    assert np.array_equal(bc, expect.astype(np.uint32))
+    # masses and energy deposit
+    assert od.field("nodalMass").sum() == pytest.approx(1.125 ** 3, rel=1e-13)
+    assert od.field("volo").sum() == pytest.approx(1.125 ** 3, rel=1e-13)
+    einit = 3.948746e+7 * (s / 45.0) ** 3
+    assert od.field("e")[0] == pytest.approx(einit, rel=1e-15)
+    h = 1.125 / s
+    assert od.get("deltatime") == pytest.approx(0.5 * h / np.sqrt(2 * einit), rel=1e-15)
+
+
+def test_cycle_one_closed_form():
+    """Cycle 1 from the initial state: p = q = ss = 0, so no force acts, no
+    node moves and vdov = 0; the EOS then gives e[1] = einit,
+    p[1] = (2/3) einit, ss[1] = sqrt(10/9 einit), ss = TINY3 elsewhere, and
+    dtcourant/dthydro stay 1e20, so cycle 2 takes dt = 1.2 dt0."""
+    od = O.OracleDomain(8)
+    x0, dt0 = od.field("x"), od.get("deltatime")
+    einit = od.field("e")[0]
+    assert od.cycle_once() == 0
+    assert np.array_equal(od.field("x"), x0) and not od.field("xd").any()
+    e, p, ss = od.field("e"), od.field("p"), od.field("ss")
+    assert e[0] == einit and p[0] == pytest.approx(2 / 3 * einit, rel=1e-15)
+    assert ss[0] == pytest.approx(np.sqrt(10 / 9 * einit), rel=1e-15)
+    assert np.all(ss[1:] == 0.333333e-18) and not od.field("q").any()
+    assert od.get("dtcourant") == 1e20 and od.get("dthydro") == 1e20
+    od.time_increment()
+    assert od.get("deltatime") == dt0 * 1.2
+
+
+def test_lulesh2_published_s30_quirk_free():
+    """With the three reference quirks cleared the arithmetic is LULESH 2.0's;
+    its published sample output for problem size 30 reads 932 iterations and
+    Final Origin Energy 2.025075e+05."""
+    od = O.OracleDomain(30, 0, 8)
+    rc, _ = od.run(10 ** 9)
+    assert rc == 0 and od.get("cycle") == 932
+    assert "%.6e" % od.verify()[0] == "2.025075e+05"
+    assert od.get("time") == pytest.approx(1e-2, abs=1e-15)
+
+
+def test_quirks_each_change_results():
+    base = O.OracleDomain(8, 0, 2)
+    base.run(150)
+    eb = base.field("e")
+    for q in (O.Q1, O.Q2, O.Q3):
+        od = O.OracleDomain(8, q, 2)
+        od.run(150)
+        assert not np.array_equal(od.field("e"), eb), q
+
+
+def test_symmetry_quirk_free_vs_reference():
+    """Quirk-free the solution is x/y symmetric up to rounding; the
+    reference's quirks break the symmetry at the percent level."""
+    a = O.OracleDomain(12, 0, 2)
+    a.run(10 ** 9)
+    b = O.OracleDomain(12, O.QUIRKS_REFERENCE, 2)
+    b.run(10 ** 9)
+    assert a.verify()[3] < 1e-10
+    assert b.verify()[3] > 1e-4
+
+
+def test_symmetry_plane_nodes_stay_on_their_planes():
+    """applyAccelerationBoundaryConditionsForNodes zeroes the normal
+    acceleration on the symmetry planes, so those nodes never leave them."""
+    od = O.OracleDomain(8, O.QUIRKS_REFERENCE, 2)
+    od.run(10 ** 9)
+    s1 = 9
+    n = np.arange(s1 ** 3)
+    col, row, plane = n % s1, (n // s1) % s1, n // (s1 * s1)
+    assert not od.field("x")[col == 0].any() and not od.field("xd")[col == 0].any()
+    assert not od.field("y")[row == 0].any() and not od.field("yd")[row == 0].any()
+    assert not od.field("z")[plane == 0].any() and not od.field("zd")[plane == 0].any()
+    assert od.field("x")[col == 8].max() > 1.125  # the free face has moved out
+
+
+def test_openmp_bitwise_equals_serial_and_reruns():
+    a = O.OracleDomain(12, O.QUIRKS_REFERENCE, 1)
+    a.run(80)
+    b = O.OracleDomain(12, O.QUIRKS_REFERENCE, 4)
+    b.run(80)
+    for f in ("e", "p", "q", "ss", "v", "x", "xd", "zd"):
+        assert np.array_equal(a.field(f).view(np.uint64), b.field(f).view(np.uint64)), f
+
+
+@pytest.mark.slow
+def test_reference_s30_does_not_reach_stoptime():
+    """The reference arithmetic (quirk Q1) at s=30: dt collapses geometrically
+    from cycle ~1350 and time stalls at 9.6672e-3 < stoptime, so the
+    reference's `while time < stoptime` loop never ends at this size."""
+    od = O.OracleDomain(30, O.QUIRKS_REFERENCE, 8)
+    rc, _ = od.run(1700)
+    assert rc == 0 and od.get("cycle") == 1700
+    assert od.get("deltatime") < 1e-10
+    assert od.get("time") == pytest.approx(9.667208e-3, rel=1e-6)
