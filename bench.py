@@ -29,16 +29,16 @@ def alg_bytes(E, N):
     every array element a kernel needs counted once, 8 B per double, 4 B per
     elemBC mask; nodal gathers count each node once)."""
     return {
-        # x,y,z,xd,yd,zd (48N) + p,q,v,volo,ss,elemMass (48E) + 6 corner arrays written (384E)
-        "k_force": 432 * E + 48 * N,
-        # 6 corner arrays read (384E) + nodalMass (8N) + x..zd read and written (96N)
-        "k_node": 384 * E + 104 * N,
+        # x,y,z,xd,yd,zd read (48N) + p,q,v,volo,ss,elemMass (48E) + nodalMass (8N)
+        # + x..zd written (48N); corner forces stay in LDS
+        "k_nodal": 48 * E + 104 * N,
         # x..zd (48N) + volo, v (16E) + vnew delv arealg vdov delx*3 delv*3 written (80E)
         "k_kin": 96 * E + 48 * N,
         # elemBC 4 + delv*3 24 + delx*3 24 + vdov elemMass volo vnew v delv e p q arealg 80
         # read, e p q ss v 40 written
         "k_qeos": 172 * E,
         "k_dt": 16 * ((E + 255) // 256),
+        "k_flip": 0,
     }
 
 
@@ -95,7 +95,7 @@ def main():
         if cnt == 0:
             continue
         avg = ms / cnt
-        gbs = ab[name] / (avg * 1e-3) / 1e9
+        gbs = ab[name] / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
         kernels[name] = {"avg_ms": round(avg, 5), "launches": cnt,
                          "alg_bytes": ab[name], "GB_s": round(gbs, 1)}
     dominant = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])

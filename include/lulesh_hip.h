@@ -121,11 +121,11 @@ int lulesh_verify(lulesh_ctx *ctx, double out[4]);
  * With timing on, every kernel launch of the context is bracketed by HIP
  * events on the context's stream; lulesh_kernel_time returns the summed
  * device time (ms) and launch count of kernel `id` since the last reset. */
-#define LULESH_K_FORCE 0  /* stress + FB hourglass corner forces */
-#define LULESH_K_NODE 1   /* corner gather, acceleration, BCs, velocity, position */
-#define LULESH_K_KIN 2    /* kinematics + MonoQ gradients */
-#define LULESH_K_QEOS 3   /* MonoQ limiter, EOS, volume update, dt partial minima */
-#define LULESH_K_DT 4     /* dt minima + next timeIncrement! */
+#define LULESH_K_NODAL 0 /* lagrangeNodal: corner forces in LDS, fixed-order gather, node update */
+#define LULESH_K_KIN 1   /* kinematics + MonoQ gradients */
+#define LULESH_K_QEOS 2  /* MonoQ limiter, EOS, volume update, dt partial minima */
+#define LULESH_K_DT 3    /* dt minima + next timeIncrement! */
+#define LULESH_K_FLIP 4  /* makes the positions k_nodal wrote current */
 #define LULESH_K_COUNT 5
 int lulesh_set_timing(lulesh_ctx *ctx, int32_t on);
 /* Device arithmetic check (no reference counterpart): evaluates, on device

@@ -22,7 +22,6 @@ struct lulesh_ctx {
   DevFields f{};
   DevScalars *sc = nullptr;   // device
   DevScalars *hsc = nullptr;  // pinned host mirror
-  double *cf = nullptr;
   double *dtpart = nullptr;
   int keep_aux = 0;
   bool partials_valid = false;
@@ -76,7 +75,6 @@ LaunchArgs make_args(lulesh_ctx *c, int gated, int loop_mode) {
   LaunchArgs a;
   a.sc = c->sc;
   a.f = c->f;
-  a.cf = c->cf;
   a.dtpart = c->dtpart;
   a.s = c->nx;
   a.E = c->E;
@@ -149,22 +147,35 @@ struct FieldRef {
   int64_t n;
 };
 
-FieldRef find_field(lulesh_ctx *c, const char *name) {
+// node positions/velocities live in the NodeSet the device parity selects
+FieldRef find_field(lulesh_ctx *c, const char *name, int parity) {
   if (!name) return {nullptr, 0};
   const std::string s(name);
   DevFields &f = c->f;
+  const NodeSet &S = f.ns[parity & 1];
 #define EF(nm) \
   if (s == #nm) return {f.nm, c->E};
 #define NF(nm) \
   if (s == #nm) return {f.nm, c->N};
+#define SF(nm) \
+  if (s == #nm) return {S.nm, c->N};
   EF(e) EF(p) EF(q) EF(ql) EF(qq) EF(v) EF(volo) EF(delv) EF(vdov) EF(arealg) EF(ss)
   EF(elemMass) EF(vnew) EF(delv_xi) EF(delv_eta) EF(delv_zeta) EF(delx_xi) EF(delx_eta)
   EF(delx_zeta) EF(dxx) EF(dyy) EF(dzz)
-  NF(x) NF(y) NF(z) NF(xd) NF(yd) NF(zd) NF(xdd) NF(ydd) NF(zdd) NF(fx) NF(fy) NF(fz)
-  NF(nodalMass)
+  SF(x) SF(y) SF(z) SF(xd) SF(yd) SF(zd)
+  NF(xdd) NF(ydd) NF(zdd) NF(fx) NF(fy) NF(fz) NF(nodalMass)
 #undef EF
 #undef NF
+#undef SF
   return {nullptr, 0};
+}
+
+// current parity (syncs the scalar mirror)
+int current_parity(lulesh_ctx *c, int *parity) {
+  HIP_TRY(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *parity = c->hsc->parity;
+  return 0;
 }
 
 double *scalar_slot(DevScalars *h, const std::string &s) {
@@ -209,7 +220,7 @@ int build_graph(lulesh_ctx *c) {
   const LaunchArgs a = make_args(c, 1, 1);
   hipGraph_t g = nullptr;
   HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  const int ids[] = {KID_FORCE, KID_NODE, KID_KIN, KID_QEOS, KID_DT};
+  const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
   for (int id : ids) {
     hipError_t e = launch_kernel(id, c->stream, a);
     if (e != hipSuccess) {
@@ -273,14 +284,14 @@ int lulesh_create(int32_t nx, int32_t nr, int32_t balance, int32_t cost, int32_t
   double **ef[] = {&f.e, &f.p, &f.q, &f.ql, &f.qq, &f.v, &f.volo, &f.delv, &f.vdov, &f.arealg,
                    &f.ss, &f.elemMass, &f.vnew, &f.delv_xi, &f.delv_eta, &f.delv_zeta,
                    &f.delx_xi, &f.delx_eta, &f.delx_zeta, &f.dxx, &f.dyy, &f.dzz};
-  double **nf[] = {&f.x, &f.y, &f.z, &f.xd, &f.yd, &f.zd, &f.xdd, &f.ydd, &f.zdd,
-                   &f.fx, &f.fy, &f.fz, &f.nodalMass};
+  double **nf[] = {&f.ns[0].x, &f.ns[0].y, &f.ns[0].z, &f.ns[0].xd, &f.ns[0].yd, &f.ns[0].zd,
+                   &f.ns[1].x, &f.ns[1].y, &f.ns[1].z, &f.ns[1].xd, &f.ns[1].yd, &f.ns[1].zd,
+                   &f.xdd, &f.ydd, &f.zdd, &f.fx, &f.fy, &f.fz, &f.nodalMass};
   for (double **p : ef)
     if (!rc) rc = dev_alloc(c, p, E);
   for (double **p : nf)
     if (!rc) rc = dev_alloc(c, p, N);
   if (!rc) rc = dev_alloc(c, &f.elemBC, E);
-  if (!rc) rc = dev_alloc(c, &c->cf, 48 * E);
   if (!rc) rc = dev_alloc(c, &c->dtpart, 2 * qeos_blocks(E));
   if (!rc) rc = dev_alloc(c, &c->sc, 1);
   if (!rc) rc = lulesh_reset(c);
@@ -317,7 +328,11 @@ const char *lulesh_last_error(const lulesh_ctx *c) { return c ? c->last_error.c_
 
 int lulesh_download(lulesh_ctx *c, const char *field, double *dst, int64_t n) {
   if (!c || !dst) return LULESH_ERR_INVALID_ARG;
-  FieldRef r = find_field(c, field);
+  HIP_TRY(c, hipSetDevice(c->device));
+  int par = 0;
+  int rc0 = current_parity(c, &par);
+  if (rc0) return rc0;
+  FieldRef r = find_field(c, field, par);
   if (!r.ptr) return fail(c, LULESH_ERR_INVALID_ARG, std::string("unknown field ") + (field ? field : "(null)"));
   if (n != r.n) return fail(c, LULESH_ERR_INVALID_ARG, "field length mismatch");
   HIP_TRY(c, hipSetDevice(c->device));
@@ -327,7 +342,11 @@ int lulesh_download(lulesh_ctx *c, const char *field, double *dst, int64_t n) {
 
 int lulesh_upload(lulesh_ctx *c, const char *field, const double *src, int64_t n) {
   if (!c || !src) return LULESH_ERR_INVALID_ARG;
-  FieldRef r = find_field(c, field);
+  HIP_TRY(c, hipSetDevice(c->device));
+  int par = 0;
+  int rc0 = current_parity(c, &par);
+  if (rc0) return rc0;
+  FieldRef r = find_field(c, field, par);
   if (!r.ptr) return fail(c, LULESH_ERR_INVALID_ARG, std::string("unknown field ") + (field ? field : "(null)"));
   if (n != r.n) return fail(c, LULESH_ERR_INVALID_ARG, "field length mismatch");
   HIP_TRY(c, hipSetDevice(c->device));
@@ -406,8 +425,8 @@ int lulesh_lagrange_nodal(lulesh_ctx *c) {
   HIP_TRY(c, hipSetDevice(c->device));
   const LaunchArgs a = make_args(c, 0, 0);
   int rc = clear_err(c);
-  if (!rc) rc = launch(c, KID_FORCE, a);
-  if (!rc) rc = launch(c, KID_NODE, a);
+  if (!rc) rc = launch(c, KID_NODAL, a);
+  if (!rc) rc = launch(c, KID_FLIP, a);
   if (rc) return rc;
   c->partials_valid = false;
   return finish(c, true);
@@ -443,7 +462,7 @@ int lulesh_lagrange_leapfrog(lulesh_ctx *c) {
   HIP_TRY(c, hipSetDevice(c->device));
   const LaunchArgs a = make_args(c, 0, 0);
   int rc = clear_err(c);
-  const int ids[] = {KID_FORCE, KID_NODE, KID_KIN, KID_QEOS, KID_DT};
+  const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
   for (int id : ids)
     if (!rc) rc = launch(c, id, a);
   if (rc) return rc;
@@ -480,7 +499,7 @@ int lulesh_run(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
       if (use_graph) {
         HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
       } else {
-        const int ids[] = {KID_FORCE, KID_NODE, KID_KIN, KID_QEOS, KID_DT};
+        const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
         for (int id : ids) {
           rc = launch(c, id, a);
           if (rc) return rc;
@@ -568,7 +587,7 @@ int lulesh_math_check(int32_t op, const double *a, const double *b, double *out,
 }
 
 const char *lulesh_kernel_name(int32_t id) {
-  static const char *names[] = {"k_force", "k_node", "k_kin", "k_qeos", "k_dt"};
+  static const char *names[] = {"k_nodal", "k_kin", "k_qeos", "k_dt", "k_flip"};
   if (id < 0 || id >= LULESH_K_COUNT) return "";
   return names[id];
 }

@@ -23,6 +23,14 @@ struct DevScalars {
   long long num_iters;  // driver loop bound of the current lulesh_run
   int err;              // lowest error code raised (ERR_NONE when clean)
   int active;           // 1 while the current cycle executes (driver loop)
+  int parity;           // which NodeSet holds the current x..zd
+  int pad_;
+};
+
+// node positions and velocities; two sets, ping-ponged by the fused nodal
+// kernel (reads set `parity`, writes the other, then parity flips)
+struct NodeSet {
+  double *x, *y, *z, *xd, *yd, *zd;
 };
 
 // structure-of-arrays FP64 fields, one device buffer each
@@ -33,15 +41,16 @@ struct DevFields {
   double *dxx, *dyy, *dzz;
   uint32_t *elemBC;
   // node
-  double *x, *y, *z, *xd, *yd, *zd, *xdd, *ydd, *zdd, *fx, *fy, *fz, *nodalMass;
+  NodeSet ns[2];
+  double *xdd, *ydd, *zdd, *fx, *fy, *fz, *nodalMass;
 };
 
 enum KernelId {
-  KID_FORCE = 0,
-  KID_NODE = 1,
-  KID_KIN = 2,
-  KID_QEOS = 3,
-  KID_DT = 4,
+  KID_NODAL = 0,
+  KID_KIN = 1,
+  KID_QEOS = 2,
+  KID_DT = 3,
+  KID_FLIP = 4,
   KID_DTPART = 5,
   KID_TIME_INCREMENT = 6,
   KID_RUN_BEGIN = 7,
@@ -51,7 +60,6 @@ enum KernelId {
 struct LaunchArgs {
   DevScalars *sc;
   DevFields f;
-  double *cf;      // 6 x 8 x E corner forces
   double *dtpart;  // 2 x qeos_blocks(E) time-constraint partial minima
   int s;
   int64_t E, N;

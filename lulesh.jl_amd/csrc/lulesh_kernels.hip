@@ -84,10 +84,11 @@ __global__ void k_init_nodes(int s, int64_t N, DevFields f) {
   const int64_t col = n % s1, rest = n / s1;
   const int64_t row = rest % s1, plane = rest / s1;
   // tx = 1.125*(colLoc*nx + col)/meshEdgeElems with colLoc = 0, tp = 1
-  f.x[n] = (1.125 * (double)col) / (double)s;
-  f.y[n] = (1.125 * (double)row) / (double)s;
-  f.z[n] = (1.125 * (double)plane) / (double)s;
-  f.xd[n] = 0.0; f.yd[n] = 0.0; f.zd[n] = 0.0;
+  const NodeSet &S = f.ns[0];
+  S.x[n] = (1.125 * (double)col) / (double)s;
+  S.y[n] = (1.125 * (double)row) / (double)s;
+  S.z[n] = (1.125 * (double)plane) / (double)s;
+  S.xd[n] = 0.0; S.yd[n] = 0.0; S.zd[n] = 0.0;
   f.xdd[n] = 0.0; f.ydd[n] = 0.0; f.zdd[n] = 0.0;
   f.fx[n] = 0.0; f.fy[n] = 0.0; f.fz[n] = 0.0;
 }
@@ -98,7 +99,7 @@ __global__ void k_init_elems(int s, int64_t E, DevFields f) {
   int64_t nid[8];
   elem_nodes(e, s, nid);
   double X[8], Y[8], Z[8];
-  gather8(f.x, nid, X); gather8(f.y, nid, Y); gather8(f.z, nid, Z);
+  gather8(f.ns[0].x, nid, X); gather8(f.ns[0].y, nid, Y); gather8(f.ns[0].z, nid, Z);
   const double vol = elem_volume(X, Y, Z);
   f.volo[e] = vol;
   f.elemMass[e] = vol;
@@ -185,6 +186,7 @@ __global__ void k_init_scalars(int s, DevScalars *sc, double *e0, const double *
   sc->err = ERR_NONE;
   sc->active = 0;
   sc->num_iters = 0;
+  sc->parity = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -205,149 +207,250 @@ __global__ void k_run_begin(DevScalars *sc, int64_t num_iters) {
   if (go) time_increment(sc);
 }
 
-// corner forces of one element.  cf layout: [component][corner][element],
-// components 0..2 stress x,y,z and 3..5 hourglass x,y,z.
-__global__ __launch_bounds__(BLOCK) void k_force(DevScalars *sc, int gated, int s, int64_t E,
-                                                 DevFields f, double *__restrict__ cf, int quirks) {
-  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (e >= E) return;
+// ---------------------------------------------------------------------------
+// lagrangeNodal (src/domain.jl:1840-1872) as one kernel.
+//
+// A workgroup owns a column of NT x NT nodes and a chunk of CZ node planes.
+// It marches up the chunk one element plane at a time: its 256 threads
+// compute the corner forces of the 16 x 16 elements around the node tile
+// (integrateStressForElems + calcFBHourglassForceForElems arithmetic), stage
+// them in LDS, and the node threads fold them into per-node running sums.
+// A node's sum takes its elements in ascending element id -- the plane below
+// first, then its own plane, each in (row, col) order -- which is exactly
+// the nodeElemCornerList order of the reference's gathers
+// (src/domain.jl:1112-1127, 1678-1693); stress and hourglass corners are
+// summed separately and added at the end, as the reference does.  Finished
+// node planes are updated at once (acceleration, symmetry BCs, velocity,
+// position; src/domain.jl:1781-1838) into the other NodeSet, so tiles that
+// still read the old positions of shared halo nodes never see new values.
+// Elements on tile edges are computed by every tile that needs them
+// (16x16x(CZ+1) per 15x15xCZ nodes): no corner force ever goes to HBM.
+// ---------------------------------------------------------------------------
+constexpr int ET = 16;  // element tile edge = threads per row
+constexpr int NT = 15;  // node tile edge
+constexpr int CZ = 16;  // node planes per chunk
+
+__global__ __launch_bounds__(BLOCK) void k_nodal(DevScalars *sc, int gated, int s, DevFields f,
+                                                 int quirks, int keep_aux, int tiles, int nblocks) {
   if (gated && !sc->active) return;
-  int64_t nid[8];
-  elem_nodes(e, s, nid);
-  double X[8], Y[8], Z[8];
-  gather8(f.x, nid, X); gather8(f.y, nid, Y); gather8(f.z, nid, Z);
+  // XCD-aware order: the dispatcher deals blocks round-robin over the 8 XCDs,
+  // so give each XCD a contiguous run of tiles (their halos share its L2)
+  const int per = (nblocks + 7) / 8;
+  const int vb = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (vb >= nblocks) return;
+  const int tx = vb % tiles, ty = (vb / tiles) % tiles, chunk = vb / (tiles * tiles);
+  const int i0 = tx * NT, j0 = ty * NT;
+  const int k0 = chunk * CZ;
+  const int k1 = min(k0 + CZ, s + 1);
+  const int t = threadIdx.x, ti = t % ET, tj = t / ET;
+  const int s1 = s + 1;
+  // this thread's element in each plane, and its node column
+  const int ei = i0 - 1 + ti, ej = j0 - 1 + tj;
+  const bool elem_col = ei >= 0 && ei < s && ej >= 0 && ej < s;
+  const int ni = i0 + ti, nj = j0 + tj;
+  const bool node_col = ti < NT && tj < NT && ni <= s && nj <= s;
+  // the four elements around the node column, in ascending id order:
+  // (ni-1,nj-1) (ni,nj-1) (ni-1,nj) (ni,nj); LDS slot and node position
+  // inside each (corner = ox + 2oy... mapped to the hex order below)
+  const bool h00 = ni >= 1 && nj >= 1, h10 = ni < s && nj >= 1;
+  const bool h01 = ni >= 1 && nj < s, h11 = ni < s && nj < s;
+  const int l00 = tj * ET + ti, l10 = l00 + 1, l01 = l00 + ET, l11 = l00 + ET + 1;
+  // corner of the node in element (ox,oy) = node offset inside it:
+  // (1,1)->2, (0,1)->3, (1,0)->1, (0,0)->0 at the bottom; +4 at the top
 
-  // integrateStressForElems, src/domain.jl:1097-1108 (sig from 963-969)
-  const double detJ = jacobian_volume(X, Y, Z);
-  if (detJ <= 0.0) raise(sc, ERR_EARLY_VOLUME);
-  {
-    double B[8][3];
-    node_normals(X, Y, Z, quirks, B);
-    const double ms = -((-f.p[e]) - f.q[e]);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int n = 0; n < 8; ++n) cf[(int64_t)(c * 8 + n) * E + e] = ms * B[n][c];
-  }
-
-  // calcHourglassControlForElems, src/domain.jl:1707-1729
-  const double v = f.v[e];
-  const double determ = f.volo[e] * v;
-  if (v <= 0.0) raise(sc, ERR_HG_VOLUME);
+  const int par = sc->parity;
+  const NodeSet in = f.ns[par], out = f.ns[par ^ 1];
   const double hourg = sc->hgcoef;
-  if (!(hourg > 0.0)) return;
-
-  // calcFBHourglassForceForElems, src/domain.jl:1505-1639
-  double hourgam[8][4];
-  {
-    double dvdx[8], dvdy[8], dvdz[8];
-    volume_derivative(X, Y, Z, quirks, dvdx, dvdy, dvdz);
-    const double volinv = 1.0 / determ;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      double hx = X[0] * gam(0, m), hy = Y[0] * gam(0, m), hz = Z[0] * gam(0, m);
-#pragma unroll
-      for (int n = 1; n < 8; ++n) {
-        hx = hx + X[n] * gam(n, m);
-        hy = hy + Y[n] * gam(n, m);
-        hz = hz + Z[n] * gam(n, m);
-      }
-#pragma unroll
-      for (int n = 0; n < 8; ++n)
-        hourgam[n][m] = gam(n, m) - volinv * (((dvdx[n] * hx) + (dvdy[n] * hy)) + (dvdz[n] * hz));
-    }
-  }
-  const double volume13 = cbrt_cr(determ);
-  const double coefficient = ((((-hourg) * 0.01) * f.ss[e]) * f.elemMass[e]) / volume13;
-  const double *vel[3] = {f.xd, f.yd, f.zd};
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double V[8];
-    gather8(vel[c], nid, V);
-    double h[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      double acc = hourgam[0][m] * V[0];
-#pragma unroll
-      for (int n = 1; n < 8; ++n) acc = acc + hourgam[n][m] * V[n];
-      h[m] = acc;
-    }
-#pragma unroll
-    for (int n = 0; n < 8; ++n)
-      cf[(int64_t)((3 + c) * 8 + n) * E + e] =
-          coefficient * ((((hourgam[n][0] * h[0]) + (hourgam[n][1] * h[1])) + (hourgam[n][2] * h[2])) +
-                         (hourgam[n][3] * h[3]));
-  }
-}
-
-// corner of element (ei,ej,ek) that node (i,j,k) occupies, by node offset
-// (i-ei) + 2(j-ej) + 4(k-ek): the hex corner order of src/domain.jl:158-165
-LDEV int corner_of(int ox, int oy, int oz) {
-  // offsets (0,0,0)->0 (1,0,0)->1 (1,1,0)->2 (0,1,0)->3, +4 for oz
-  return 4 * oz + (oy ? (ox ? 2 : 3) : (ox ? 1 : 0));
-}
-
-// gathers of src/domain.jl:1112-1127 and 1678-1693, then
-// calcAccelerationForNodes .. calcPositionForNodes (1781-1838)
-__global__ __launch_bounds__(BLOCK) void k_node(DevScalars *sc, int gated, int s, int64_t N,
-                                                int64_t E, DevFields f, const double *__restrict__ cf,
-                                                int keep_aux) {
-  const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (n >= N) return;
-  if (gated && !sc->active) return;
-  const int64_t s1 = s + 1;
-  const int i = (int)(n % s1), j = (int)((n / s1) % s1), k = (int)(n / (s1 * s1));
-  double fs0 = 0.0, fs1 = 0.0, fs2 = 0.0, fh0 = 0.0, fh1 = 0.0, fh2 = 0.0;
-  for (int dk = -1; dk <= 0; ++dk) {
-    const int ek = k + dk;
-    if (ek < 0 || ek >= s) continue;
-    for (int dj = -1; dj <= 0; ++dj) {
-      const int ej = j + dj;
-      if (ej < 0 || ej >= s) continue;
-      for (int di = -1; di <= 0; ++di) {
-        const int ei = i + di;
-        if (ei < 0 || ei >= s) continue;
-        const int64_t e = ((int64_t)ek * s + ej) * s + ei;
-        const int c = corner_of(-di, -dj, -dk);
-        fs0 = fs0 + cf[(int64_t)(0 * 8 + c) * E + e];
-        fs1 = fs1 + cf[(int64_t)(1 * 8 + c) * E + e];
-        fs2 = fs2 + cf[(int64_t)(2 * 8 + c) * E + e];
-        fh0 = fh0 + cf[(int64_t)(3 * 8 + c) * E + e];
-        fh1 = fh1 + cf[(int64_t)(4 * 8 + c) * E + e];
-        fh2 = fh2 + cf[(int64_t)(5 * 8 + c) * E + e];
-      }
-    }
-  }
-  double fx = fs0, fy = fs1, fz = fs2;
-  if (sc->hgcoef > 0.0) {
-    fx = fx + fh0;
-    fy = fy + fh1;
-    fz = fz + fh2;
-  }
-  const double mass = f.nodalMass[n];
-  double xdd = fx / mass, ydd = fy / mass, zdd = fz / mass;
-  // applyAccelerationBoundaryConditionsForNodes: symmX/Y/Z are the col 0,
-  // row 0 and plane 0 node sets (src/domain.jl:773-791)
-  if (i == 0) xdd = 0.0;
-  if (j == 0) ydd = 0.0;
-  if (k == 0) zdd = 0.0;
+  const bool hg_on = hourg > 0.0;
   const double dt = sc->deltatime, u_cut = sc->u_cut;
-  double xd = f.xd[n] + xdd * dt;
-  if (fabs(xd) < u_cut) xd = 0.0;
-  double yd = f.yd[n] + ydd * dt;
-  if (fabs(yd) < u_cut) yd = 0.0;
-  double zd = f.zd[n] + zdd * dt;
-  if (fabs(zd) < u_cut) zd = 0.0;
-  f.xd[n] = xd;
-  f.yd[n] = yd;
-  f.zd[n] = zd;
-  f.x[n] = f.x[n] + xd * dt;
-  f.y[n] = f.y[n] + yd * dt;
-  f.z[n] = f.z[n] + zd * dt;
-  if (keep_aux) {
-    f.fx[n] = fx; f.fy[n] = fy; f.fz[n] = fz;
-    f.xdd[n] = xdd; f.ydd[n] = ydd; f.zdd[n] = zdd;
+
+  __shared__ double lds[24 * BLOCK];  // [corner][component][element slot]
+
+  double cs0 = 0.0, cs1 = 0.0, cs2 = 0.0, ch0 = 0.0, ch1 = 0.0, ch2 = 0.0;  // node plane ek
+  double ns0 = 0.0, ns1 = 0.0, ns2 = 0.0, nh0 = 0.0, nh1 = 0.0, nh2 = 0.0;  // node plane ek+1
+
+  for (int ek = k0 - 1; ek < k1; ++ek) {
+    const bool plane = ek >= 0 && ek < s;
+    double hgf[8][3];
+    if (plane && elem_col) {
+      const int64_t e = ((int64_t)ek * s + ej) * s + ei;
+      int64_t nid[8];
+      elem_nodes(e, s, nid);
+      double X[8], Y[8], Z[8];
+      gather8(in.x, nid, X); gather8(in.y, nid, Y); gather8(in.z, nid, Z);
+      // stress corners, src/domain.jl:1097-1108 (sig from 963-969)
+      const double detJ = jacobian_volume(X, Y, Z);
+      if (detJ <= 0.0) raise(sc, ERR_EARLY_VOLUME);
+      {
+        double B[8][3];
+        node_normals(X, Y, Z, quirks, B);
+        const double ms = -((-f.p[e]) - f.q[e]);
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) lds[(n * 3 + c) * BLOCK + t] = ms * B[n][c];
+      }
+      // hourglass corners, src/domain.jl:1707-1729 and 1505-1639
+      const double v = f.v[e];
+      const double determ = f.volo[e] * v;
+      if (v <= 0.0) raise(sc, ERR_HG_VOLUME);
+      if (hg_on) {
+        double hourgam[8][4];
+        {
+          double dvdx[8], dvdy[8], dvdz[8];
+          volume_derivative(X, Y, Z, quirks, dvdx, dvdy, dvdz);
+          const double volinv = 1.0 / determ;
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            double hx = X[0] * gam(0, m), hy = Y[0] * gam(0, m), hz = Z[0] * gam(0, m);
+#pragma unroll
+            for (int n = 1; n < 8; ++n) {
+              hx = hx + X[n] * gam(n, m);
+              hy = hy + Y[n] * gam(n, m);
+              hz = hz + Z[n] * gam(n, m);
+            }
+#pragma unroll
+            for (int n = 0; n < 8; ++n)
+              hourgam[n][m] =
+                  gam(n, m) - volinv * (((dvdx[n] * hx) + (dvdy[n] * hy)) + (dvdz[n] * hz));
+          }
+        }
+        const double volume13 = cbrt_cr(determ);
+        const double coefficient = ((((-hourg) * 0.01) * f.ss[e]) * f.elemMass[e]) / volume13;
+        const double *vel[3] = {in.xd, in.yd, in.zd};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          double V[8];
+          gather8(vel[c], nid, V);
+          double h[4];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            double acc = hourgam[0][m] * V[0];
+#pragma unroll
+            for (int n = 1; n < 8; ++n) acc = acc + hourgam[n][m] * V[n];
+            h[m] = acc;
+          }
+#pragma unroll
+          for (int n = 0; n < 8; ++n)
+            hgf[n][c] = coefficient * ((((hourgam[n][0] * h[0]) + (hourgam[n][1] * h[1])) +
+                                        (hourgam[n][2] * h[2])) +
+                                       (hourgam[n][3] * h[3]));
+        }
+      }
+    }
+    __syncthreads();
+    // stress: finish node plane ek with the bottom corners of plane ek, start
+    // node plane ek+1 with its top corners (plane ek precedes plane ek+1)
+    if (node_col && plane) {
+      if (ek >= k0) {
+#define LULESH_ACC(flag, slot, corner)                                  \
+  if (flag) {                                                           \
+    cs0 = cs0 + lds[((corner) * 3 + 0) * BLOCK + (slot)];               \
+    cs1 = cs1 + lds[((corner) * 3 + 1) * BLOCK + (slot)];               \
+    cs2 = cs2 + lds[((corner) * 3 + 2) * BLOCK + (slot)];               \
   }
+        LULESH_ACC(h00, l00, 2) LULESH_ACC(h10, l10, 3) LULESH_ACC(h01, l01, 1)
+        LULESH_ACC(h11, l11, 0)
+#undef LULESH_ACC
+      }
+      if (ek + 1 < k1) {
+#define LULESH_ACC(flag, slot, corner)                                  \
+  if (flag) {                                                           \
+    ns0 = ns0 + lds[((corner) * 3 + 0) * BLOCK + (slot)];               \
+    ns1 = ns1 + lds[((corner) * 3 + 1) * BLOCK + (slot)];               \
+    ns2 = ns2 + lds[((corner) * 3 + 2) * BLOCK + (slot)];               \
+  }
+        LULESH_ACC(h00, l00, 6) LULESH_ACC(h10, l10, 7) LULESH_ACC(h01, l01, 5)
+        LULESH_ACC(h11, l11, 4)
+#undef LULESH_ACC
+      }
+    }
+    __syncthreads();
+    if (hg_on) {
+      if (plane && elem_col) {
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) lds[(n * 3 + c) * BLOCK + t] = hgf[n][c];
+      }
+      __syncthreads();
+      if (node_col && plane) {
+        if (ek >= k0) {
+#define LULESH_ACC(flag, slot, corner)                                  \
+  if (flag) {                                                           \
+    ch0 = ch0 + lds[((corner) * 3 + 0) * BLOCK + (slot)];               \
+    ch1 = ch1 + lds[((corner) * 3 + 1) * BLOCK + (slot)];               \
+    ch2 = ch2 + lds[((corner) * 3 + 2) * BLOCK + (slot)];               \
+  }
+          LULESH_ACC(h00, l00, 2) LULESH_ACC(h10, l10, 3) LULESH_ACC(h01, l01, 1)
+          LULESH_ACC(h11, l11, 0)
+#undef LULESH_ACC
+        }
+        if (ek + 1 < k1) {
+#define LULESH_ACC(flag, slot, corner)                                  \
+  if (flag) {                                                           \
+    nh0 = nh0 + lds[((corner) * 3 + 0) * BLOCK + (slot)];               \
+    nh1 = nh1 + lds[((corner) * 3 + 1) * BLOCK + (slot)];               \
+    nh2 = nh2 + lds[((corner) * 3 + 2) * BLOCK + (slot)];               \
+  }
+          LULESH_ACC(h00, l00, 6) LULESH_ACC(h10, l10, 7) LULESH_ACC(h01, l01, 5)
+          LULESH_ACC(h11, l11, 4)
+#undef LULESH_ACC
+        }
+      }
+      __syncthreads();
+    }
+    // node plane ek is complete: calcAccelerationForNodes ..
+    // calcPositionForNodes for it
+    if (node_col && ek >= k0) {
+      const int64_t n = ((int64_t)ek * s1 + nj) * s1 + ni;
+      double fx = cs0, fy = cs1, fz = cs2;
+      if (hg_on) {
+        fx = fx + ch0;
+        fy = fy + ch1;
+        fz = fz + ch2;
+      }
+      const double mass = f.nodalMass[n];
+      double xdd = fx / mass, ydd = fy / mass, zdd = fz / mass;
+      // symmetry planes: the col 0, row 0 and plane 0 node sets
+      // (src/domain.jl:773-791, 1787-1806)
+      if (ni == 0) xdd = 0.0;
+      if (nj == 0) ydd = 0.0;
+      if (ek == 0) zdd = 0.0;
+      double xd = in.xd[n] + xdd * dt;
+      if (fabs(xd) < u_cut) xd = 0.0;
+      double yd = in.yd[n] + ydd * dt;
+      if (fabs(yd) < u_cut) yd = 0.0;
+      double zd = in.zd[n] + zdd * dt;
+      if (fabs(zd) < u_cut) zd = 0.0;
+      out.xd[n] = xd;
+      out.yd[n] = yd;
+      out.zd[n] = zd;
+      out.x[n] = in.x[n] + xd * dt;
+      out.y[n] = in.y[n] + yd * dt;
+      out.z[n] = in.z[n] + zd * dt;
+      if (keep_aux) {
+        f.fx[n] = fx; f.fy[n] = fy; f.fz[n] = fz;
+        f.xdd[n] = xdd; f.ydd[n] = ydd; f.zdd[n] = zdd;
+      }
+    }
+    cs0 = ns0; cs1 = ns1; cs2 = ns2;
+    ch0 = nh0; ch1 = nh1; ch2 = nh2;
+    ns0 = ns1 = ns2 = nh0 = nh1 = nh2 = 0.0;
+  }
+}
+
+// the positions written by k_nodal become the current set
+__global__ void k_flip(DevScalars *sc, int gated) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (gated && !sc->active) return;
+  sc->parity ^= 1;
+}
+
+void nodal_grid(int s, int &tiles, int &nblocks) {
+  tiles = (s + 1 + NT - 1) / NT;
+  const int chunks = (s + 1 + CZ - 1) / CZ;
+  nblocks = tiles * tiles * chunks;
 }
 
 // calcKinematicsForElems (src/domain.jl:2099-2147), the deviatoric loop
@@ -360,8 +463,9 @@ __global__ __launch_bounds__(BLOCK) void k_kin(DevScalars *sc, int gated, int s,
   int64_t nid[8];
   elem_nodes(e, s, nid);
   double X[8], Y[8], Z[8], XV[8], YV[8], ZV[8];
-  gather8(f.x, nid, X); gather8(f.y, nid, Y); gather8(f.z, nid, Z);
-  gather8(f.xd, nid, XV); gather8(f.yd, nid, YV); gather8(f.zd, nid, ZV);
+  const NodeSet &S = f.ns[sc->parity];
+  gather8(S.x, nid, X); gather8(S.y, nid, Y); gather8(S.z, nid, Z);
+  gather8(S.xd, nid, XV); gather8(S.yd, nid, YV); gather8(S.zd, nid, ZV);
   const double volo = f.volo[e];
   const double volume = elem_volume(X, Y, Z);
   const double vnew = volume / volo;
@@ -762,13 +866,16 @@ hipError_t launch_math(hipStream_t st, int op, const double *a, const double *b,
 
 hipError_t launch_kernel(int id, hipStream_t st, const LaunchArgs &a) {
   switch (id) {
-    case KID_FORCE:
-      hipLaunchKernelGGL(k_force, dim3(nblk(a.E, BLOCK)), dim3(BLOCK), 0, st, a.sc, a.gated, a.s,
-                         a.E, a.f, a.cf, a.quirks);
+    case KID_NODAL: {
+      int tiles, nb;
+      nodal_grid(a.s, tiles, nb);
+      const int grid = ((nb + 7) / 8) * 8;
+      hipLaunchKernelGGL(k_nodal, dim3(grid), dim3(BLOCK), 0, st, a.sc, a.gated, a.s, a.f,
+                         a.quirks, a.keep_aux, tiles, nb);
       break;
-    case KID_NODE:
-      hipLaunchKernelGGL(k_node, dim3(nblk(a.N, BLOCK)), dim3(BLOCK), 0, st, a.sc, a.gated, a.s,
-                         a.N, a.E, a.f, a.cf, a.keep_aux);
+    }
+    case KID_FLIP:
+      hipLaunchKernelGGL(k_flip, dim3(1), dim3(64), 0, st, a.sc, a.gated);
       break;
     case KID_KIN:
       hipLaunchKernelGGL(k_kin, dim3(nblk(a.E, BLOCK)), dim3(BLOCK), 0, st, a.sc, a.gated, a.s,

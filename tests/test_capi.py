@@ -52,7 +52,7 @@ def test_argument_validation_before_device_use():
     assert not ctx.value
     assert lib.lulesh_math_check(7, None, None, None, 1, 0) == 100
     assert lib.lulesh_destroy(None) == 100
-    assert lib.lulesh_kernel_name(0).decode() == "k_force"
+    assert lib.lulesh_kernel_name(0).decode() == "k_nodal"
 
 
 def test_missing_library_fails_loudly(monkeypatch):
