@@ -42,6 +42,60 @@ def alg_bytes(E, N):
     }
 
 
+def bench_adjoint(nx=48, cycles=200, every=20):
+    """Adjoint/forward wall-time ratio: the gradient of the total internal
+    energy after `cycles` cycles (hand-written adjoint kernels, checkpoints
+    every `every` cycles, including its own forward pass) over the plain
+    forward run of the same cycles, both from the initial state."""
+    from lulesh_amd import Domain, LuleshProblem, grad_run, run
+    d = Domain(LuleshProblem(cycles, True, nx, 11, 1, 1))
+    run(d, 5)
+    grad_run(d, 5, 5)
+    d.reset()
+    t0 = time.perf_counter()
+    run(d, cycles)
+    fwd = time.perf_counter() - t0
+    d.reset()
+    t0 = time.perf_counter()
+    g = grad_run(d, cycles, every)
+    grad = time.perf_counter() - t0
+    d.close()
+    return {"config": f"s={nx}, {cycles} cycles, checkpoint every {every}",
+            "forward_ms": round(fwd * 1e3, 3), "gradient_ms": round(grad * 1e3, 3),
+            "ratio": round(grad / fwd, 3), "loss": g["loss"]}
+
+
+def bench_batched(n=64, seed=0, cap=5000):
+    """Batched speedup: 64 uneven Sedov instances (the seeded ragged
+    workload) run to stop time (at most `cap` cycles each) one after another
+    on the GPU, over the same instances in one batch."""
+    from lulesh_amd import Batch, c5_instances
+    nx, scale, hg, nr = c5_instances(n, seed)
+    lone = [Batch([nx[i]], [scale[i]], [hg[i]], [nr[i]]) for i in range(n)]
+    for b in lone:
+        b.run(2)
+        b.views[0].reset()
+    t0 = time.perf_counter()
+    lone_cycles = [int(b.run(cap)[0]) for b in lone]
+    seq = time.perf_counter() - t0
+    for b in lone:
+        b.close()
+    batch = Batch(nx, scale, hg, nr)
+    batch.run(2)
+    batch.views[0].reset()
+    t0 = time.perf_counter()
+    cyc = batch.run(cap)
+    bt = time.perf_counter() - t0
+    batch.close()
+    if [int(c) for c in cyc] != lone_cycles:
+        raise RuntimeError("batched cycle counts differ from the sequential runs")
+    return {"config": f"{n} Sedov instances (seed {seed}), s in [{min(nx)}, {max(nx)}], "
+                      f"{sum(v ** 3 for v in nx)} elements, to stoptime or {cap} cycles",
+            "sequential_ms": round(seq * 1e3, 2), "batched_ms": round(bt * 1e3, 2),
+            "speedup": round(seq / bt, 3), "cycles_min": min(lone_cycles),
+            "cycles_max": max(lone_cycles), "reached_stoptime": int(sum(c < cap for c in lone_cycles))}
+
+
 def cpu_baseline(nx, cycles, nthreads):
     import oracle as O
     od = O.OracleDomain(nx, O.QUIRKS_REFERENCE, nthreads)
@@ -65,10 +119,25 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (the "
+                         "reference point for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 8}))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs one single-GPU domain; multi-GPU (--gpus > 1) is not "
-              "part of it (see DESIGN.md for how the domain would shard).", file=sys.stderr)
+              "part of it.", file=sys.stderr)
         sys.exit(2)
 
     from lulesh_amd import Domain, LuleshProblem, run
@@ -112,6 +181,9 @@ def main():
                 "frac": round(dk["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": dominant}
 
+    adjoint = None if args.no_adjoint else bench_adjoint()
+    batched = None if args.no_batched else bench_batched()
+
     cpu = None
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -122,7 +194,7 @@ def main():
                          f"same workload, {secs:.1f} s, {args.cpu_threads} thread(s)"}
 
     out = {
-        "metric": "FP64 zone-cycles/s at s=192",
+        "metric": "FP64 zone-cycles/s at s=192; adjoint/forward time ratio; batched speedup",
         "value": round(value, 1),
         "unit": "zone-cycles/s",
         "n_gpus": 1,
@@ -139,6 +211,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "adjoint": adjoint,
+        "batched": batched,
         "alg_bytes_per_cycle": sum(ab.values()),
     }
     print(json.dumps(out))

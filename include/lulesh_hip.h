@@ -117,6 +117,53 @@ int lulesh_run(lulesh_ctx *ctx, int64_t num_iters, int64_t *cycles_run);
  * out = {e[1], MaxAbsDiff, TotalAbsDiff, MaxRelDiff} over plane 0 */
 int lulesh_verify(lulesh_ctx *ctx, double out[4]);
 
+/* ---- ragged batches -----------------------------------------------------
+ * No reference counterpart (the reference runs one Domain per process).  A
+ * batch holds n independent Sedov domains, laid out back to back in the
+ * same flat SoA buffers, each with its own size nx[i], deposited energy
+ * einit_scale[i] x the reference's einit (dt0 follows from it), initial
+ * hgcoef[i] and region count nr[i] (NULL arrays take the reference values
+ * 1.0, 3.0 and 1).  Every cycle-level entry point above advances all
+ * instances with one launch per kernel and per-instance dt; an instance
+ * whose driver loop has ended is skipped, so its state stays exactly what
+ * a lone run leaves.  The single-domain entry points address instance 0. */
+int lulesh_batch_create(int32_t n, const int32_t *nx, const double *einit_scale,
+                        const double *hgcoef, const int32_t *nr, int32_t quirks, int32_t device,
+                        lulesh_ctx **out);
+int32_t lulesh_batch_size(const lulesh_ctx *ctx);
+int64_t lulesh_batch_num_elem(const lulesh_ctx *ctx, int32_t inst);
+int64_t lulesh_batch_num_node(const lulesh_ctx *ctx, int32_t inst);
+int lulesh_batch_download(lulesh_ctx *ctx, int32_t inst, const char *field, double *dst,
+                          int64_t n);
+int lulesh_batch_upload(lulesh_ctx *ctx, int32_t inst, const char *field, const double *src,
+                        int64_t n);
+int lulesh_batch_download_elem_bc(lulesh_ctx *ctx, int32_t inst, uint32_t *dst, int64_t n);
+int lulesh_batch_upload_elem_bc(lulesh_ctx *ctx, int32_t inst, const uint32_t *src, int64_t n);
+int lulesh_batch_get_scalar(lulesh_ctx *ctx, int32_t inst, const char *name, double *value);
+int lulesh_batch_set_scalar(lulesh_ctx *ctx, int32_t inst, const char *name, double value);
+/* the driver loop for every instance at once; cycles_run[i] per instance */
+int lulesh_batch_run(lulesh_ctx *ctx, int64_t num_iters, int64_t *cycles_run);
+int lulesh_batch_verify(lulesh_ctx *ctx, int32_t inst, double out[4]);
+
+/* ---- differentiation ----------------------------------------------------
+ * Replaces the reference's Enzyme reverse-mode call on lagrangeLeapFrog
+ * (examples/benchmark.jl:48-51,85-86) with hand-written adjoint kernels.
+ * lulesh_grad_run runs `ncycles` cycles of the driver loop from the current
+ * state (checkpointing it every `checkpoint_every` cycles), then sweeps back
+ * through them, recomputing each segment from its checkpoint, and returns
+ *   *loss        = sum_e e[e] * elemMass[e] at the end (total internal energy),
+ *   grad_e[E]    = d loss / d e at the start,
+ *   grad_params  = d loss / d {hgcoef, qlc_monoq, qqc_monoq},
+ *   dt_schedule  = the ncycles time steps the forward pass took (may be NULL).
+ * The time step is held fixed at the recorded schedule (an input leaf, as
+ * timeIncrement! is outside the differentiated call in the reference); the
+ * context is left in the forward pass's end state.  One-domain contexts only.
+ * lulesh_replay advances the state `ncycles` cycles with those time steps
+ * instead of timeIncrement! (for finite-difference checks of the gradient). */
+int lulesh_grad_run(lulesh_ctx *ctx, int64_t ncycles, int64_t checkpoint_every, double *grad_e,
+                    double *grad_params, double *loss, double *dt_schedule);
+int lulesh_replay(lulesh_ctx *ctx, const double *dt, int64_t ncycles);
+
 /* ---- measurement --------------------------------------------------------
  * With timing on, every kernel launch of the context is bracketed by HIP
  * events on the context's stream; lulesh_kernel_time returns the summed
@@ -128,6 +175,15 @@ int lulesh_verify(lulesh_ctx *ctx, double out[4]);
 #define LULESH_K_FLIP 4  /* makes the positions k_nodal wrote current */
 #define LULESH_K_COUNT 5
 int lulesh_set_timing(lulesh_ctx *ctx, int32_t on);
+/* k_nodal geometry (no reference counterpart): each workgroup marches
+ * `depth` node planes of its 15x15 node column.  0 restores the default
+ * rule (deepest of 16/8/4/2 that still launches >= 1024 workgroups).
+ * lulesh_autotune times k_nodal on the live state for depths 2..32 (the
+ * kernel only writes the inactive node set, so the state is unchanged) and
+ * keeps the fastest; *chosen and *best_ms (per launch) may be NULL. */
+int lulesh_set_nodal_depth(lulesh_ctx *ctx, int32_t depth);
+int32_t lulesh_nodal_depth(const lulesh_ctx *ctx);
+int lulesh_autotune(lulesh_ctx *ctx, int32_t *chosen, double *best_ms);
 /* Device arithmetic check (no reference counterpart): evaluates, on device
  * `device`, the scalar routines every kernel relies on, so tests can pin
  * their rounding: op 0 = cbrt(a) (correctly rounded), 1 = sqrt(a),

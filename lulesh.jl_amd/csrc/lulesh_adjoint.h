@@ -1,0 +1,35 @@
+// Arguments and launchers of the adjoint kernels (single-domain contexts).
+#pragma once
+
+#include "lulesh_kernels.h"
+
+namespace lulesh {
+
+struct AdjArgs {
+  DevScalars *sc;
+  DevFields f;       // instance 0 (offsets are zero for a one-instance context)
+  NodeSet cur, old;  // node set holding x', xd' and the one holding S_c
+  int s;
+  int64_t E, N;
+  bool q1, q2, q3;
+  // adjoint of the cycle state (S_{c+1} on entry, S_c on exit)
+  double *lx, *ly, *lz, *lxd, *lyd, *lzd;  // [N]
+  double *le, *lp, *lq, *lv, *lss;         // [E]
+  double *lF;                              // [3][N] force adjoint per node
+  double *nbr;                             // [6][E] delv adjoint sent to the read neighbour
+  double *ldvs, *ldx;                      // [3][E] own delv / delx adjoints
+  double *lvnew, *lvnew_delv;              // [E]
+  double *corner;                          // [6][8][E] per-corner node adjoints
+  double *pacc;                            // [3][E] hgcoef, qlc_monoq, qqc_monoq partials
+};
+
+enum AdjKernel { AK_QEOS = 0, AK_KIN, AK_NODE1, AK_FORCE, AK_NODE2, AK_SEED };
+
+hipError_t launch_adjoint(int which, hipStream_t st, const AdjArgs &a);
+hipError_t launch_set_dt(hipStream_t st, DevScalars *sc, const double *dt, int64_t base);
+hipError_t launch_record_dt(hipStream_t st, DevScalars *sc, double *dt, int64_t base);
+hipError_t launch_state_copy(hipStream_t st, DevScalars *sc, const DevFields &f, int64_t E,
+                             int64_t N, double *buf, int to_buf);
+hipError_t launch_set_cycle(hipStream_t st, DevScalars *sc, long long cycle);
+
+}  // namespace lulesh

@@ -1,6 +1,11 @@
 // C ABI of liblulesh_hip (declared in include/lulesh_hip.h): context
 // lifetime, Domain field access, the hot-path entry points, the device-side
-// driver loop (graph replay) and per-kernel event timing.
+// driver loop (graph replay), ragged batches and per-kernel event timing.
+//
+// A context holds a batch of one or more structured Sedov domains laid out
+// back to back in flat SoA device buffers.  The single-domain entry points
+// address instance 0 (their only instance); the lulesh_batch_* ones take an
+// instance index.  Cycle-level entry points always advance every instance.
 #include "lulesh_hip.h"
 
 #include <hip/hip_runtime.h>
@@ -11,20 +16,37 @@
 #include <string>
 #include <vector>
 
+#include "lulesh_adjoint.h"
 #include "lulesh_kernels.h"
 
 using namespace lulesh;
 
+// device buffers of the adjoint driver (allocated on first use)
+struct AdjBuffers {
+  bool ready = false;
+  AdjArgs a{};
+  double *dt = nullptr;        // recorded dt schedule
+  int64_t dt_cap = 0;
+  double *ckpt = nullptr;      // checkpoints, each 6N + 5E doubles
+  int64_t ckpt_cap = 0;        // states
+  double *seg = nullptr;       // states of the segment being reversed
+  int64_t seg_cap = 0;
+  double *final_state = nullptr;
+  DevScalars *final_sc = nullptr;
+  std::vector<void *> mem;
+};
+
 struct lulesh_ctx {
-  int nx = 0, nr = 1, balance = 1, cost = 1, quirks = LULESH_QUIRKS_REFERENCE, device = 0;
-  int64_t E = 0, N = 0;
+  int n = 0;  // instances
+  int quirks = LULESH_QUIRKS_REFERENCE, device = 0;
+  std::vector<InstDesc> inst;
+  std::vector<int32_t> nr, balance, cost;
+  int64_t E_total = 0, N_total = 0;
   hipStream_t stream = nullptr;
-  DevFields f{};
-  DevScalars *sc = nullptr;   // device
-  DevScalars *hsc = nullptr;  // pinned host mirror
-  double *dtpart = nullptr;
-  int keep_aux = 0;
+  Batch b{};                      // device pointers + launch geometry
+  DevScalars *hsc = nullptr;      // pinned host mirror [n]
   bool partials_valid = false;
+  int keep_aux = 0;
   // timing
   bool timing = false;
   struct Pending {
@@ -35,11 +57,12 @@ struct lulesh_ctx {
   std::vector<hipEvent_t> event_pool;
   double ktime[LULESH_K_COUNT] = {0};
   int64_t kcount[LULESH_K_COUNT] = {0};
-  // driver-loop graph (one cycle)
+  // driver-loop graph (one cycle of every instance)
   hipGraphExec_t graph = nullptr;
   int graph_keep_aux = -1;
   std::string last_error;
   std::vector<void *> allocations;
+  AdjBuffers adj;
 };
 
 namespace {
@@ -65,25 +88,18 @@ int hip_fail(lulesh_ctx *c, hipError_t e, const char *what) {
   return fail(c, LULESH_ERR_HIP, m);
 }
 
-#define HIP_TRY(c, expr)                                 \
-  do {                                                   \
-    hipError_t _e = (expr);                              \
+#define HIP_TRY(c, expr)                                   \
+  do {                                                     \
+    hipError_t _e = (expr);                                \
     if (_e != hipSuccess) return hip_fail((c), _e, #expr); \
   } while (0)
 
-LaunchArgs make_args(lulesh_ctx *c, int gated, int loop_mode) {
-  LaunchArgs a;
-  a.sc = c->sc;
-  a.f = c->f;
-  a.dtpart = c->dtpart;
-  a.s = c->nx;
-  a.E = c->E;
-  a.N = c->N;
-  a.quirks = c->quirks;
+Batch make_args(lulesh_ctx *c, int gated, int loop_mode, int64_t num_iters = 0) {
+  Batch a = c->b;
   a.gated = gated;
-  a.keep_aux = c->keep_aux;
   a.loop_mode = loop_mode;
-  a.num_iters = 0;
+  a.keep_aux = c->keep_aux;
+  a.num_iters = num_iters;
   return a;
 }
 
@@ -99,7 +115,7 @@ hipEvent_t get_event(lulesh_ctx *c) {
 }
 
 // one launch on the context stream, bracketed by events when timing is on
-int launch(lulesh_ctx *c, int id, const LaunchArgs &a) {
+int launch(lulesh_ctx *c, int id, const Batch &a) {
   hipEvent_t ea = nullptr, eb = nullptr;
   const bool timed = c->timing && id < LULESH_K_COUNT;
   if (timed) {
@@ -117,9 +133,19 @@ int launch(lulesh_ctx *c, int id, const LaunchArgs &a) {
   return 0;
 }
 
+int pull_scalars(lulesh_ctx *c) {
+  HIP_TRY(c, hipMemcpyAsync(c->hsc, c->b.sc, sizeof(DevScalars) * c->n, hipMemcpyDeviceToHost,
+                            c->stream));
+  return 0;
+}
+
 // synchronise, fold pending event pairs into the timing totals, then report
-// the lowest physics error raised on the device (if any)
+// the lowest physics error raised on the device (if any, over all instances)
 int finish(lulesh_ctx *c, bool check_err) {
+  if (check_err) {
+    int rc = pull_scalars(c);
+    if (rc) return rc;
+  }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   for (auto &p : c->pending) {
     float ms = 0.f;
@@ -131,8 +157,8 @@ int finish(lulesh_ctx *c, bool check_err) {
   }
   c->pending.clear();
   if (!check_err) return 0;
-  HIP_TRY(c, hipMemcpy(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost));
-  const int err = c->hsc->err;
+  int err = ERR_NONE;
+  for (int i = 0; i < c->n; ++i) err = c->hsc[i].err < err ? c->hsc[i].err : err;
   if (err != ERR_NONE) {
     if (err >= 1 && err <= 9) return fail(c, err, kErrText[err]);
     return fail(c, LULESH_ERR_HIP, "unknown device error code");
@@ -147,18 +173,20 @@ struct FieldRef {
   int64_t n;
 };
 
-// node positions/velocities live in the NodeSet the device parity selects
-FieldRef find_field(lulesh_ctx *c, const char *name, int parity) {
-  if (!name) return {nullptr, 0};
+// field of one instance; node positions/velocities live in the NodeSet the
+// instance's parity selects
+FieldRef find_field(lulesh_ctx *c, int inst, const char *name, int parity) {
+  if (!name || inst < 0 || inst >= c->n) return {nullptr, 0};
   const std::string s(name);
-  DevFields &f = c->f;
+  const DevFields &f = c->b.f;
+  const InstDesc &I = c->inst[inst];
   const NodeSet &S = f.ns[parity & 1];
 #define EF(nm) \
-  if (s == #nm) return {f.nm, c->E};
+  if (s == #nm) return {f.nm + I.e0, I.E};
 #define NF(nm) \
-  if (s == #nm) return {f.nm, c->N};
+  if (s == #nm) return {f.nm + I.n0, I.N};
 #define SF(nm) \
-  if (s == #nm) return {S.nm, c->N};
+  if (s == #nm) return {S.nm + I.n0, I.N};
   EF(e) EF(p) EF(q) EF(ql) EF(qq) EF(v) EF(volo) EF(delv) EF(vdov) EF(arealg) EF(ss)
   EF(elemMass) EF(vnew) EF(delv_xi) EF(delv_eta) EF(delv_zeta) EF(delx_xi) EF(delx_eta)
   EF(delx_zeta) EF(dxx) EF(dyy) EF(dzz)
@@ -168,14 +196,6 @@ FieldRef find_field(lulesh_ctx *c, const char *name, int parity) {
 #undef NF
 #undef SF
   return {nullptr, 0};
-}
-
-// current parity (syncs the scalar mirror)
-int current_parity(lulesh_ctx *c, int *parity) {
-  HIP_TRY(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  *parity = c->hsc->parity;
-  return 0;
 }
 
 double *scalar_slot(DevScalars *h, const std::string &s) {
@@ -192,7 +212,7 @@ double *scalar_slot(DevScalars *h, const std::string &s) {
 template <typename T>
 int dev_alloc(lulesh_ctx *c, T **p, int64_t n) {
   void *q = nullptr;
-  hipError_t e = hipMalloc(&q, (size_t)n * sizeof(T));
+  hipError_t e = hipMalloc(&q, (size_t)(n > 0 ? n : 1) * sizeof(T));
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
   c->allocations.push_back(q);
   *p = (T *)q;
@@ -207,6 +227,7 @@ void release(lulesh_ctx *c) {
   }
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   for (void *p : c->allocations) (void)hipFree(p);
+  for (void *p : c->adj.mem) (void)hipFree(p);
   if (c->hsc) (void)hipHostFree(c->hsc);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -217,7 +238,7 @@ int build_graph(lulesh_ctx *c) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
   }
-  const LaunchArgs a = make_args(c, 1, 1);
+  const Batch a = make_args(c, 1, 1);
   hipGraph_t g = nullptr;
   HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
@@ -240,9 +261,460 @@ int build_graph(lulesh_ctx *c) {
   return 0;
 }
 
+// k_nodal geometry: every instance marches `cz` node planes per workgroup.
+// cz <= 0 picks the default: the deepest of 16, 8, 4, 2 that still gives
+// the launch >= 1024 workgroups (small domains trade some recomputed halo
+// planes for parallelism).  Uploads the instance and block tables.
+int plan_nodal(lulesh_ctx *c, int cz) {
+  if (cz <= 0) {
+    cz = 16;
+    for (;;) {
+      int64_t total = 0;
+      for (const InstDesc &I : c->inst) total += nodal_blocks(I.s, cz);
+      if (total >= 1024 || cz <= 2) break;
+      cz /= 2;
+    }
+  }
+  std::vector<int> gblock_inst;
+  int gb = 0;
+  for (int i = 0; i < c->n; ++i) {
+    InstDesc &I = c->inst[i];
+    I.cz = cz;
+    I.gb0 = gb;
+    I.gbn = nodal_blocks(I.s, cz);
+    gb += I.gbn;
+    gblock_inst.insert(gblock_inst.end(), I.gbn, i);
+  }
+  c->b.gblocks = gb;
+  HIP_TRY(c, hipMemcpyAsync((void *)c->b.inst, c->inst.data(), sizeof(InstDesc) * c->n,
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync((void *)c->b.gblock_inst, gblock_inst.data(), sizeof(int) * gb,
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  return 0;
+}
+
+int check_inst(lulesh_ctx *c, int32_t inst) {
+  if (!c) return LULESH_ERR_INVALID_ARG;
+  if (inst < 0 || inst >= c->n) return fail(c, LULESH_ERR_INVALID_ARG, "instance out of range");
+  return 0;
+}
+
+int create(int32_t n, const int32_t *nx, const double *einit_scale, const double *hgcoef,
+           const int32_t *nr, const int32_t *balance, const int32_t *cost, int32_t quirks,
+           int32_t device, lulesh_ctx **out) {
+  if (!out) return LULESH_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (n < 1 || !nx || quirks < 0 || quirks > LULESH_QUIRKS_REFERENCE) return LULESH_ERR_INVALID_ARG;
+  for (int i = 0; i < n; ++i) {
+    if (nx[i] < 1 || nx[i] > 1200) return LULESH_ERR_INVALID_ARG;
+    if (einit_scale && !(einit_scale[i] > 0.0)) return LULESH_ERR_INVALID_ARG;
+  }
+  lulesh_ctx *c = new lulesh_ctx();
+  c->n = n;
+  c->quirks = quirks;
+  c->device = device;
+  c->inst.resize(n);
+  int eb = 0, nb = 0, gb_max = 0;
+  int64_t e0 = 0, n0 = 0;
+  std::vector<int> eblock_inst, nblock_inst;
+  for (int i = 0; i < n; ++i) {
+    InstDesc &I = c->inst[i];
+    const int s = nx[i];
+    I.s = s;
+    I.tiles = nodal_tiles(s);
+    I.E = (int64_t)s * s * s;
+    I.N = (int64_t)(s + 1) * (s + 1) * (s + 1);
+    I.e0 = e0;
+    I.n0 = n0;
+    I.eb0 = eb;
+    I.ebn = (int)((I.E + BLOCK - 1) / BLOCK);
+    I.nb0 = nb;
+    I.nbn = (int)((I.N + BLOCK - 1) / BLOCK);
+    I.gb0 = 0;
+    I.gbn = 0;
+    I.cz = 0;
+    I.einit_scale = einit_scale ? einit_scale[i] : 1.0;
+    I.hgcoef = hgcoef ? hgcoef[i] : 3.0;
+    e0 += I.E;
+    n0 += I.N;
+    eb += I.ebn;
+    nb += I.nbn;
+    gb_max += nodal_blocks(s, 1);
+    eblock_inst.insert(eblock_inst.end(), I.ebn, i);
+    nblock_inst.insert(nblock_inst.end(), I.nbn, i);
+    c->nr.push_back(nr ? nr[i] : 1);
+    c->balance.push_back(balance ? balance[i] : 1);
+    c->cost.push_back(cost ? cost[i] : 1);
+  }
+  c->E_total = e0;
+  c->N_total = n0;
+  Batch &B = c->b;
+  B.n = n;
+  B.eblocks = eb;
+  B.nblocks = nb;
+  B.gblocks = 0;
+  B.quirks = quirks;
+  int rc = 0;
+  hipError_t he = hipSetDevice(device);
+  if (he != hipSuccess) rc = hip_fail(c, he, "hipSetDevice");
+  if (!rc) {
+    he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) rc = hip_fail(c, he, "hipStreamCreate");
+  }
+  if (!rc) {
+    he = hipHostMalloc((void **)&c->hsc, sizeof(DevScalars) * n, hipHostMallocDefault);
+    if (he != hipSuccess) rc = hip_fail(c, he, "hipHostMalloc");
+  }
+  DevFields &f = B.f;
+  double **ef[] = {&f.e, &f.p, &f.q, &f.ql, &f.qq, &f.v, &f.volo, &f.delv, &f.vdov, &f.arealg,
+                   &f.ss, &f.elemMass, &f.vnew, &f.delv_xi, &f.delv_eta, &f.delv_zeta,
+                   &f.delx_xi, &f.delx_eta, &f.delx_zeta, &f.dxx, &f.dyy, &f.dzz};
+  double **nf[] = {&f.ns[0].x, &f.ns[0].y, &f.ns[0].z, &f.ns[0].xd, &f.ns[0].yd, &f.ns[0].zd,
+                   &f.ns[1].x, &f.ns[1].y, &f.ns[1].z, &f.ns[1].xd, &f.ns[1].yd, &f.ns[1].zd,
+                   &f.xdd, &f.ydd, &f.zdd, &f.fx, &f.fy, &f.fz, &f.nodalMass};
+  for (double **p : ef)
+    if (!rc) rc = dev_alloc(c, p, e0);
+  for (double **p : nf)
+    if (!rc) rc = dev_alloc(c, p, n0);
+  if (!rc) rc = dev_alloc(c, &f.elemBC, e0);
+  if (!rc) rc = dev_alloc(c, &B.dtpart, 2 * (int64_t)eb);
+  if (!rc) rc = dev_alloc(c, &B.sc, n);
+  InstDesc *dinst = nullptr;
+  int *deb = nullptr, *dnb = nullptr, *dgb = nullptr;
+  if (!rc) rc = dev_alloc(c, &dinst, n);
+  if (!rc) rc = dev_alloc(c, &deb, eb);
+  if (!rc) rc = dev_alloc(c, &dnb, nb);
+  if (!rc) rc = dev_alloc(c, &dgb, gb_max);
+  if (!rc) {
+    he = hipMemcpy(deb, eblock_inst.data(), sizeof(int) * eb, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(dnb, nblock_inst.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
+    if (he != hipSuccess) rc = hip_fail(c, he, "hipMemcpy (batch tables)");
+  }
+  B.inst = dinst;
+  B.eblock_inst = deb;
+  B.nblock_inst = dnb;
+  B.gblock_inst = dgb;
+  if (!rc) rc = plan_nodal(c, 0);
+  if (!rc) rc = lulesh_reset(c);
+  if (rc) {
+    release(c);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+// Fetch the instance's scalars (synchronous).
+int get_inst_scalars(lulesh_ctx *c, int32_t inst, DevScalars *out) {
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMemcpyAsync(c->hsc + inst, c->b.sc + inst, sizeof(DevScalars),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *out = c->hsc[inst];
+  return 0;
+}
+
+int run_cycles(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run /* [n] or null */) {
+  HIP_TRY(c, hipSetDevice(c->device));
+  int rc = pull_scalars(c);
+  if (!rc) rc = finish(c, false);
+  if (rc) return rc;
+  std::vector<long long> cycle0(c->n);
+  long long min_cycle0 = 0;
+  for (int i = 0; i < c->n; ++i) {
+    cycle0[i] = c->hsc[i].cycle;
+    min_cycle0 = (i == 0 || cycle0[i] < min_cycle0) ? cycle0[i] : min_cycle0;
+  }
+  const Batch a = make_args(c, 1, 1, num_iters);
+  rc = launch(c, KID_RUN_BEGIN, a);
+  if (rc) return rc;
+  // with timing on, kernels are launched one by one so each can carry its
+  // events; otherwise one captured cycle (all instances) is replayed
+  const bool use_graph = !c->timing;
+  if (use_graph) {
+    rc = build_graph(c);
+    if (rc) return rc;
+  }
+  // replay exactly the cycles num_iters still allows the least advanced
+  // active instance (its first cycle always runs); only stop-time exits and
+  // unequal instances leave gated no-op replays behind
+  int64_t cap = 8;
+  long long next_cycle = min_cycle0 + 1;
+  for (;;) {
+    const long long left = (long long)num_iters - next_cycle + 1;
+    const int64_t batch = left < 1 ? 1 : (left < cap ? left : cap);
+    for (int64_t i = 0; i < batch; ++i) {
+      if (use_graph) {
+        HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+      } else {
+        const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
+        for (int id : ids) {
+          rc = launch(c, id, a);
+          if (rc) return rc;
+        }
+      }
+    }
+    rc = pull_scalars(c);
+    if (!rc) rc = finish(c, false);
+    if (rc) return rc;
+    bool any = false;
+    long long mn = 0;
+    for (int i = 0; i < c->n; ++i)
+      if (c->hsc[i].active) {
+        if (!any || c->hsc[i].cycle < mn) mn = c->hsc[i].cycle;
+        any = true;
+      }
+    if (!any) break;
+    next_cycle = mn;
+    if (cap < 256) cap *= 2;
+  }
+  c->partials_valid = false;
+  int err = ERR_NONE;
+  for (int i = 0; i < c->n; ++i) {
+    if (cycles_run) cycles_run[i] = (int64_t)(c->hsc[i].cycle - cycle0[i]);
+    err = c->hsc[i].err < err ? c->hsc[i].err : err;
+  }
+  if (err != ERR_NONE) {
+    if (err >= 1 && err <= 9) return fail(c, err, kErrText[err]);
+    return fail(c, LULESH_ERR_HIP, "unknown device error code");
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// adjoint driver
+// ---------------------------------------------------------------------------
+int adj_alloc(lulesh_ctx *c, double **p, int64_t n) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, (size_t)(n > 0 ? n : 1) * sizeof(double));
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc (adjoint)");
+  c->adj.mem.push_back(q);
+  *p = (double *)q;
+  return 0;
+}
+
+int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
+  AdjBuffers &B = c->adj;
+  const int64_t E = c->inst[0].E, N = c->inst[0].N;
+  const int64_t state = 6 * N + 5 * E;
+  int rc = 0;
+  if (!B.ready) {
+    AdjArgs &a = B.a;
+    a.sc = c->b.sc;
+    a.f = c->b.f;
+    a.s = c->inst[0].s;
+    a.E = E;
+    a.N = N;
+    a.q1 = (c->quirks & LULESH_Q1_FACE5_NORMAL) != 0;
+    a.q2 = (c->quirks & LULESH_Q2_DVDX_SWAP) != 0;
+    a.q3 = (c->quirks & LULESH_Q3_XIP_SELF) != 0;
+    double **nv[] = {&a.lx, &a.ly, &a.lz, &a.lxd, &a.lyd, &a.lzd};
+    double **ev[] = {&a.le, &a.lp, &a.lq, &a.lv, &a.lss, &a.lvnew, &a.lvnew_delv};
+    for (double **p : nv)
+      if (!rc) rc = adj_alloc(c, p, N);
+    for (double **p : ev)
+      if (!rc) rc = adj_alloc(c, p, E);
+    if (!rc) rc = adj_alloc(c, &a.lF, 3 * N);
+    if (!rc) rc = adj_alloc(c, &a.nbr, 6 * E);
+    if (!rc) rc = adj_alloc(c, &a.ldvs, 3 * E);
+    if (!rc) rc = adj_alloc(c, &a.ldx, 3 * E);
+    if (!rc) rc = adj_alloc(c, &a.corner, 48 * E);
+    if (!rc) rc = adj_alloc(c, &a.pacc, 3 * E);
+    if (!rc) rc = adj_alloc(c, &B.final_state, state);
+    if (!rc) {
+      hipError_t he = hipMalloc((void **)&B.final_sc, sizeof(DevScalars));
+      if (he != hipSuccess) return hip_fail(c, he, "hipMalloc (adjoint)");
+      B.mem.push_back(B.final_sc);
+    }
+    if (rc) return rc;
+    B.ready = true;
+  }
+  if (ncycles > B.dt_cap) {
+    rc = adj_alloc(c, &B.dt, ncycles);
+    if (rc) return rc;
+    B.dt_cap = ncycles;
+  }
+  const int64_t nck = (ncycles + every - 1) / every;
+  if (nck > B.ckpt_cap) {
+    rc = adj_alloc(c, &B.ckpt, nck * state);
+    if (rc) return rc;
+    B.ckpt_cap = nck;
+  }
+  if (every > B.seg_cap) {
+    rc = adj_alloc(c, &B.seg, every * state);
+    if (rc) return rc;
+    B.seg_cap = every;
+  }
+  return 0;
+}
+
+int state_copy(lulesh_ctx *c, double *buf, int to_buf) {
+  hipError_t e = launch_state_copy(c->stream, c->b.sc, c->b.f, c->inst[0].E, c->inst[0].N, buf,
+                                   to_buf);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "state copy");
+}
+
+// forward cycle with dt from the recorded schedule (timeIncrement! replaced)
+int replay_cycle(lulesh_ctx *c, const double *dt, int64_t base, bool constraints) {
+  const Batch a = make_args(c, 0, 0);
+  hipError_t e = launch_set_dt(c->stream, c->b.sc, dt, base);
+  if (e != hipSuccess) return hip_fail(c, e, "set dt");
+  const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
+  for (int id : ids) {
+    if (id == KID_DT && !constraints) break;
+    int rc = launch(c, id, a);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+AdjArgs adj_args(lulesh_ctx *c, int parity) {
+  AdjArgs a = c->adj.a;
+  a.cur = c->b.f.ns[parity & 1];
+  a.old = c->b.f.ns[(parity & 1) ^ 1];
+  return a;
+}
+
+int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, double grad_params[3],
+             double *loss, double *dt_out) {
+  if (c->n != 1) return fail(c, LULESH_ERR_UNSUPPORTED, "the adjoint needs a one-domain context");
+  if (ncycles < 1 || every < 1) return fail(c, LULESH_ERR_INVALID_ARG, "ncycles and checkpoint interval must be >= 1");
+  HIP_TRY(c, hipSetDevice(c->device));
+  int rc = adj_setup(c, ncycles, every);
+  if (rc) return rc;
+  AdjBuffers &B = c->adj;
+  const int64_t E = c->inst[0].E, N = c->inst[0].N, state = 6 * N + 5 * E;
+  rc = pull_scalars(c);
+  if (!rc) rc = finish(c, false);
+  if (rc) return rc;
+  const long long cycle0 = c->hsc[0].cycle;
+  const Batch a = make_args(c, 0, 0);
+  // 1. forward pass: checkpoints every `every` cycles, dt schedule recorded
+  rc = clear_err(c);
+  for (int64_t k = 0; k < ncycles && !rc; ++k) {
+    if (k % every == 0) rc = state_copy(c, B.ckpt + (k / every) * state, 1);
+    if (!rc) rc = launch(c, KID_TIME_INCREMENT, a);
+    if (!rc) {
+      hipError_t e = launch_record_dt(c->stream, c->b.sc, B.dt, cycle0);
+      if (e != hipSuccess) rc = hip_fail(c, e, "record dt");
+    }
+    const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
+    for (int id : ids)
+      if (!rc) rc = launch(c, id, a);
+  }
+  if (!rc) rc = finish(c, true);
+  if (rc) return rc;
+  if (dt_out) HIP_TRY(c, hipMemcpy(dt_out, B.dt, sizeof(double) * ncycles, hipMemcpyDeviceToHost));
+  // loss = sum_e e * elemMass, summed on the host in element order
+  {
+    std::vector<double> ev(E), mv(E);
+    HIP_TRY(c, hipMemcpy(ev.data(), c->b.f.e, sizeof(double) * E, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(mv.data(), c->b.f.elemMass, sizeof(double) * E, hipMemcpyDeviceToHost));
+    double L = 0.0;
+    for (int64_t i = 0; i < E; ++i) L += ev[i] * mv[i];
+    if (loss) *loss = L;
+  }
+  rc = state_copy(c, B.final_state, 1);
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(B.final_sc, c->b.sc, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream));
+  // 2. reverse sweep, segment by segment
+  {
+    hipError_t e = launch_adjoint(AK_SEED, c->stream, adj_args(c, 0));
+    if (e != hipSuccess) return hip_fail(c, e, "adjoint seed");
+  }
+  const int64_t nseg = (ncycles + every - 1) / every;
+  for (int64_t sg = nseg - 1; sg >= 0 && !rc; --sg) {
+    const int64_t k0 = sg * every, k1 = (k0 + every < ncycles) ? k0 + every : ncycles;
+    rc = state_copy(c, B.ckpt + sg * state, 0);
+    if (!rc) {
+      hipError_t e = launch_set_cycle(c->stream, c->b.sc, cycle0 + k0);
+      if (e != hipSuccess) rc = hip_fail(c, e, "set cycle");
+    }
+    for (int64_t k = k0; k < k1 && !rc; ++k) {
+      rc = state_copy(c, B.seg + (k - k0) * state, 1);
+      if (!rc) rc = replay_cycle(c, B.dt, cycle0, false);
+    }
+    for (int64_t k = k1 - 1; k >= k0 && !rc; --k) {
+      // S_k back in place, then the forward nodal + kinematics of cycle k
+      rc = state_copy(c, B.seg + (k - k0) * state, 0);
+      if (!rc) {
+        hipError_t e = launch_set_cycle(c->stream, c->b.sc, cycle0 + k);
+        if (e == hipSuccess) e = launch_set_dt(c->stream, c->b.sc, B.dt, cycle0);
+        if (e != hipSuccess) rc = hip_fail(c, e, "set dt");
+      }
+      if (!rc) rc = launch(c, KID_NODAL, a);
+      if (!rc) rc = launch(c, KID_FLIP, a);
+      if (!rc) rc = launch(c, KID_KIN, a);
+      if (rc) break;
+      // the parity after the flip is needed on the host to hand the kernels
+      // the right node sets
+      rc = pull_scalars(c);
+      if (!rc) rc = finish(c, false);
+      if (rc) break;
+      const AdjArgs aa = adj_args(c, c->hsc[0].parity);
+      const int ks[] = {AK_QEOS, AK_KIN, AK_NODE1, AK_FORCE, AK_NODE2};
+      for (int kk : ks) {
+        hipError_t e = launch_adjoint(kk, c->stream, aa);
+        if (e != hipSuccess) {
+          rc = hip_fail(c, e, "adjoint kernel");
+          break;
+        }
+      }
+    }
+  }
+  if (rc) return rc;
+  // 3. results, then the state the forward pass ended in
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (grad_e) HIP_TRY(c, hipMemcpy(grad_e, B.a.le, sizeof(double) * E, hipMemcpyDeviceToHost));
+  if (grad_params) {
+    std::vector<double> pa(3 * E);
+    HIP_TRY(c, hipMemcpy(pa.data(), B.a.pacc, sizeof(double) * 3 * E, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) {
+      double sum = 0.0;
+      for (int64_t i = 0; i < E; ++i) sum += pa[k * E + i];
+      grad_params[k] = sum;
+    }
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->b.sc, B.final_sc, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream));
+  rc = state_copy(c, B.final_state, 0);
+  if (!rc) rc = finish(c, false);
+  c->partials_valid = false;
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
+
+int lulesh_grad_run(lulesh_ctx *c, int64_t ncycles, int64_t checkpoint_every, double *grad_e,
+                    double *grad_params, double *loss, double *dt_schedule) {
+  if (!c) return LULESH_ERR_INVALID_ARG;
+  return grad_run(c, ncycles, checkpoint_every, grad_e, grad_params, loss, dt_schedule);
+}
+
+int lulesh_replay(lulesh_ctx *c, const double *dt, int64_t ncycles) {
+  if (!c || !dt || ncycles < 0) return LULESH_ERR_INVALID_ARG;
+  if (c->n != 1) return fail(c, LULESH_ERR_UNSUPPORTED, "replay needs a one-domain context");
+  HIP_TRY(c, hipSetDevice(c->device));
+  int rc = adj_setup(c, ncycles > 0 ? ncycles : 1, 1);
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpy(c->adj.dt, dt, sizeof(double) * ncycles, hipMemcpyHostToDevice));
+  rc = pull_scalars(c);
+  if (!rc) rc = finish(c, false);
+  if (rc) return rc;
+  const long long cycle0 = c->hsc[0].cycle;
+  rc = clear_err(c);
+  for (int64_t k = 0; k < ncycles && !rc; ++k) rc = replay_cycle(c, c->adj.dt, cycle0, true);
+  if (rc) return rc;
+  c->partials_valid = false;
+  return finish(c, true);
+}
 
 const char *lulesh_error_string(int code) {
   if (code == LULESH_OK) return "ok";
@@ -255,54 +727,16 @@ const char *lulesh_error_string(int code) {
 
 int lulesh_create(int32_t nx, int32_t nr, int32_t balance, int32_t cost, int32_t quirks,
                   int32_t device, lulesh_ctx **out) {
-  if (!out) return LULESH_ERR_INVALID_ARG;
-  *out = nullptr;
-  if (nx < 1 || nx > 1200 || quirks < 0 || quirks > LULESH_QUIRKS_REFERENCE)
-    return LULESH_ERR_INVALID_ARG;
-  lulesh_ctx *c = new lulesh_ctx();
-  c->nx = nx;
-  c->nr = nr;
-  c->balance = balance;
-  c->cost = cost;
-  c->quirks = quirks;
-  c->device = device;
-  c->E = (int64_t)nx * nx * nx;
-  c->N = (int64_t)(nx + 1) * (nx + 1) * (nx + 1);
-  int rc = 0;
-  hipError_t he = hipSetDevice(device);
-  if (he != hipSuccess) rc = hip_fail(c, he, "hipSetDevice");
-  if (!rc) {
-    he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (he != hipSuccess) rc = hip_fail(c, he, "hipStreamCreate");
-  }
-  if (!rc) {
-    he = hipHostMalloc((void **)&c->hsc, sizeof(DevScalars), hipHostMallocDefault);
-    if (he != hipSuccess) rc = hip_fail(c, he, "hipHostMalloc");
-  }
-  DevFields &f = c->f;
-  const int64_t E = c->E, N = c->N;
-  double **ef[] = {&f.e, &f.p, &f.q, &f.ql, &f.qq, &f.v, &f.volo, &f.delv, &f.vdov, &f.arealg,
-                   &f.ss, &f.elemMass, &f.vnew, &f.delv_xi, &f.delv_eta, &f.delv_zeta,
-                   &f.delx_xi, &f.delx_eta, &f.delx_zeta, &f.dxx, &f.dyy, &f.dzz};
-  double **nf[] = {&f.ns[0].x, &f.ns[0].y, &f.ns[0].z, &f.ns[0].xd, &f.ns[0].yd, &f.ns[0].zd,
-                   &f.ns[1].x, &f.ns[1].y, &f.ns[1].z, &f.ns[1].xd, &f.ns[1].yd, &f.ns[1].zd,
-                   &f.xdd, &f.ydd, &f.zdd, &f.fx, &f.fy, &f.fz, &f.nodalMass};
-  for (double **p : ef)
-    if (!rc) rc = dev_alloc(c, p, E);
-  for (double **p : nf)
-    if (!rc) rc = dev_alloc(c, p, N);
-  if (!rc) rc = dev_alloc(c, &f.elemBC, E);
-  if (!rc) rc = dev_alloc(c, &c->dtpart, 2 * qeos_blocks(E));
-  if (!rc) rc = dev_alloc(c, &c->sc, 1);
-  if (!rc) rc = lulesh_reset(c);
-  if (rc) {
-    release(c);
-    delete c;
-    return rc;
-  }
-  *out = c;
-  return 0;
+  return create(1, &nx, nullptr, nullptr, &nr, &balance, &cost, quirks, device, out);
 }
+
+int lulesh_batch_create(int32_t n, const int32_t *nx, const double *einit_scale,
+                        const double *hgcoef, const int32_t *nr, int32_t quirks, int32_t device,
+                        lulesh_ctx **out) {
+  return create(n, nx, einit_scale, hgcoef, nr, nullptr, nullptr, quirks, device, out);
+}
+
+int32_t lulesh_batch_size(const lulesh_ctx *c) { return c ? c->n : -1; }
 
 int lulesh_destroy(lulesh_ctx *c) {
   if (!c) return LULESH_ERR_INVALID_ARG;
@@ -316,94 +750,132 @@ int lulesh_destroy(lulesh_ctx *c) {
 int lulesh_reset(lulesh_ctx *c) {
   if (!c) return LULESH_ERR_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
-  hipError_t e = launch_init(c->stream, c->nx, c->E, c->N, c->f, c->sc);
+  hipError_t e = launch_init(c->stream, c->b);
   if (e != hipSuccess) return hip_fail(c, e, "init kernels");
   c->partials_valid = false;
   return finish(c, false);
 }
 
-int64_t lulesh_num_elem(const lulesh_ctx *c) { return c ? c->E : -1; }
-int64_t lulesh_num_node(const lulesh_ctx *c) { return c ? c->N : -1; }
+int64_t lulesh_num_elem(const lulesh_ctx *c) { return c ? c->inst[0].E : -1; }
+int64_t lulesh_num_node(const lulesh_ctx *c) { return c ? c->inst[0].N : -1; }
+int64_t lulesh_batch_num_elem(const lulesh_ctx *c, int32_t inst) {
+  return (c && inst >= 0 && inst < c->n) ? c->inst[inst].E : -1;
+}
+int64_t lulesh_batch_num_node(const lulesh_ctx *c, int32_t inst) {
+  return (c && inst >= 0 && inst < c->n) ? c->inst[inst].N : -1;
+}
 const char *lulesh_last_error(const lulesh_ctx *c) { return c ? c->last_error.c_str() : "null context"; }
 
-int lulesh_download(lulesh_ctx *c, const char *field, double *dst, int64_t n) {
-  if (!c || !dst) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  int par = 0;
-  int rc0 = current_parity(c, &par);
-  if (rc0) return rc0;
-  FieldRef r = find_field(c, field, par);
+int lulesh_batch_download(lulesh_ctx *c, int32_t inst, const char *field, double *dst, int64_t n) {
+  int rc = check_inst(c, inst);
+  if (rc) return rc;
+  if (!dst) return LULESH_ERR_INVALID_ARG;
+  DevScalars s;
+  rc = get_inst_scalars(c, inst, &s);
+  if (rc) return rc;
+  FieldRef r = find_field(c, inst, field, s.parity);
   if (!r.ptr) return fail(c, LULESH_ERR_INVALID_ARG, std::string("unknown field ") + (field ? field : "(null)"));
   if (n != r.n) return fail(c, LULESH_ERR_INVALID_ARG, "field length mismatch");
-  HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipMemcpyAsync(dst, r.ptr, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   return finish(c, false);
 }
 
-int lulesh_upload(lulesh_ctx *c, const char *field, const double *src, int64_t n) {
-  if (!c || !src) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  int par = 0;
-  int rc0 = current_parity(c, &par);
-  if (rc0) return rc0;
-  FieldRef r = find_field(c, field, par);
+int lulesh_batch_upload(lulesh_ctx *c, int32_t inst, const char *field, const double *src, int64_t n) {
+  int rc = check_inst(c, inst);
+  if (rc) return rc;
+  if (!src) return LULESH_ERR_INVALID_ARG;
+  DevScalars s;
+  rc = get_inst_scalars(c, inst, &s);
+  if (rc) return rc;
+  FieldRef r = find_field(c, inst, field, s.parity);
   if (!r.ptr) return fail(c, LULESH_ERR_INVALID_ARG, std::string("unknown field ") + (field ? field : "(null)"));
   if (n != r.n) return fail(c, LULESH_ERR_INVALID_ARG, "field length mismatch");
-  HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipMemcpyAsync(r.ptr, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   c->partials_valid = false;
   return finish(c, false);
 }
 
-int lulesh_download_elem_bc(lulesh_ctx *c, uint32_t *dst, int64_t n) {
-  if (!c || !dst || n != c->E) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpyAsync(dst, c->f.elemBC, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
-  return finish(c, false);
+int lulesh_download(lulesh_ctx *c, const char *field, double *dst, int64_t n) {
+  return lulesh_batch_download(c, 0, field, dst, n);
+}
+int lulesh_upload(lulesh_ctx *c, const char *field, const double *src, int64_t n) {
+  return lulesh_batch_upload(c, 0, field, src, n);
 }
 
-int lulesh_upload_elem_bc(lulesh_ctx *c, const uint32_t *src, int64_t n) {
-  if (!c || !src || n != c->E) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpyAsync(c->f.elemBC, src, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-  return finish(c, false);
-}
-
-int lulesh_get_scalar(lulesh_ctx *c, const char *name, double *value) {
-  if (!c || !name || !value) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-  int rc = finish(c, false);
+int lulesh_batch_download_elem_bc(lulesh_ctx *c, int32_t inst, uint32_t *dst, int64_t n) {
+  int rc = check_inst(c, inst);
   if (rc) return rc;
-  const std::string s(name);
-  if (s == "cycle") {
-    *value = (double)c->hsc->cycle;
+  if (!dst || n != c->inst[inst].E) return fail(c, LULESH_ERR_INVALID_ARG, "elemBC length mismatch");
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMemcpyAsync(dst, c->b.f.elemBC + c->inst[inst].e0, (size_t)n * 4,
+                            hipMemcpyDeviceToHost, c->stream));
+  return finish(c, false);
+}
+
+int lulesh_batch_upload_elem_bc(lulesh_ctx *c, int32_t inst, const uint32_t *src, int64_t n) {
+  int rc = check_inst(c, inst);
+  if (rc) return rc;
+  if (!src || n != c->inst[inst].E) return fail(c, LULESH_ERR_INVALID_ARG, "elemBC length mismatch");
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMemcpyAsync(c->b.f.elemBC + c->inst[inst].e0, src, (size_t)n * 4,
+                            hipMemcpyHostToDevice, c->stream));
+  return finish(c, false);
+}
+
+int lulesh_download_elem_bc(lulesh_ctx *c, uint32_t *dst, int64_t n) {
+  return lulesh_batch_download_elem_bc(c, 0, dst, n);
+}
+int lulesh_upload_elem_bc(lulesh_ctx *c, const uint32_t *src, int64_t n) {
+  return lulesh_batch_upload_elem_bc(c, 0, src, n);
+}
+
+int lulesh_batch_get_scalar(lulesh_ctx *c, int32_t inst, const char *name, double *value) {
+  int rc = check_inst(c, inst);
+  if (rc) return rc;
+  if (!name || !value) return LULESH_ERR_INVALID_ARG;
+  DevScalars s;
+  rc = get_inst_scalars(c, inst, &s);
+  if (rc) return rc;
+  const std::string nm(name);
+  if (nm == "cycle") {
+    *value = (double)s.cycle;
     return 0;
   }
-  double *slot = scalar_slot(c->hsc, s);
-  if (!slot) return fail(c, LULESH_ERR_INVALID_ARG, "unknown scalar " + s);
+  double *slot = scalar_slot(&s, nm);
+  if (!slot) return fail(c, LULESH_ERR_INVALID_ARG, "unknown scalar " + nm);
   *value = *slot;
   return 0;
 }
 
-int lulesh_set_scalar(lulesh_ctx *c, const char *name, double value) {
-  if (!c || !name) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-  int rc = finish(c, false);
+int lulesh_batch_set_scalar(lulesh_ctx *c, int32_t inst, const char *name, double value) {
+  int rc = check_inst(c, inst);
   if (rc) return rc;
-  const std::string s(name);
-  if (s == "cycle") {
-    if (!(value >= 0.0) || value != std::floor(value)) return fail(c, LULESH_ERR_INVALID_ARG, "cycle must be a non-negative integer");
-    c->hsc->cycle = (long long)value;
+  if (!name) return LULESH_ERR_INVALID_ARG;
+  DevScalars s;
+  rc = get_inst_scalars(c, inst, &s);
+  if (rc) return rc;
+  const std::string nm(name);
+  if (nm == "cycle") {
+    if (!(value >= 0.0) || value != std::floor(value))
+      return fail(c, LULESH_ERR_INVALID_ARG, "cycle must be a non-negative integer");
+    s.cycle = (long long)value;
   } else {
-    double *slot = scalar_slot(c->hsc, s);
-    if (!slot) return fail(c, LULESH_ERR_INVALID_ARG, "unknown scalar " + s);
+    double *slot = scalar_slot(&s, nm);
+    if (!slot) return fail(c, LULESH_ERR_INVALID_ARG, "unknown scalar " + nm);
     *slot = value;
   }
-  HIP_TRY(c, hipMemcpyAsync(c->sc, c->hsc, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream));
+  c->hsc[inst] = s;
+  HIP_TRY(c, hipMemcpyAsync(c->b.sc + inst, c->hsc + inst, sizeof(DevScalars),
+                            hipMemcpyHostToDevice, c->stream));
   c->partials_valid = false;
   return finish(c, false);
+}
+
+int lulesh_get_scalar(lulesh_ctx *c, const char *name, double *value) {
+  return lulesh_batch_get_scalar(c, 0, name, value);
+}
+int lulesh_set_scalar(lulesh_ctx *c, const char *name, double value) {
+  return lulesh_batch_set_scalar(c, 0, name, value);
 }
 
 int lulesh_set_keep_aux(lulesh_ctx *c, int32_t on) {
@@ -423,7 +895,7 @@ int lulesh_time_increment(lulesh_ctx *c) {
 int lulesh_lagrange_nodal(lulesh_ctx *c) {
   if (!c) return LULESH_ERR_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
-  const LaunchArgs a = make_args(c, 0, 0);
+  const Batch a = make_args(c, 0, 0);
   int rc = clear_err(c);
   if (!rc) rc = launch(c, KID_NODAL, a);
   if (!rc) rc = launch(c, KID_FLIP, a);
@@ -435,7 +907,7 @@ int lulesh_lagrange_nodal(lulesh_ctx *c) {
 int lulesh_lagrange_elements(lulesh_ctx *c) {
   if (!c) return LULESH_ERR_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
-  const LaunchArgs a = make_args(c, 0, 0);
+  const Batch a = make_args(c, 0, 0);
   int rc = clear_err(c);
   if (!rc) rc = launch(c, KID_KIN, a);
   if (!rc) rc = launch(c, KID_QEOS, a);
@@ -448,7 +920,7 @@ int lulesh_lagrange_elements(lulesh_ctx *c) {
 int lulesh_calc_time_constraints(lulesh_ctx *c) {
   if (!c) return LULESH_ERR_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
-  const LaunchArgs a = make_args(c, 0, 0);
+  const Batch a = make_args(c, 0, 0);
   int rc = 0;
   if (!c->partials_valid) rc = launch(c, KID_DTPART, a);
   if (!rc) rc = launch(c, KID_DT, a);
@@ -460,7 +932,7 @@ int lulesh_calc_time_constraints(lulesh_ctx *c) {
 int lulesh_lagrange_leapfrog(lulesh_ctx *c) {
   if (!c) return LULESH_ERR_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
-  const LaunchArgs a = make_args(c, 0, 0);
+  const Batch a = make_args(c, 0, 0);
   int rc = clear_err(c);
   const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
   for (int id : ids)
@@ -472,64 +944,27 @@ int lulesh_lagrange_leapfrog(lulesh_ctx *c) {
 
 int lulesh_run(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
   if (!c) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-  int rc = finish(c, false);
-  if (rc) return rc;
-  const long long cycle0 = c->hsc->cycle;
-  LaunchArgs a = make_args(c, 1, 1);
-  a.num_iters = num_iters;
-  rc = launch(c, KID_RUN_BEGIN, a);
-  if (rc) return rc;
-  // with timing on, kernels are launched one by one so each can carry its
-  // events; otherwise one captured cycle is replayed
-  const bool use_graph = !c->timing;
-  if (use_graph) {
-    rc = build_graph(c);
-    if (rc) return rc;
-  }
-  // replay exactly the cycles num_iters still allows (the first one always
-  // runs); only a stop-time exit leaves gated no-op replays behind
-  int64_t cap = 8;
-  long long next_cycle = cycle0 + 1;
-  for (;;) {
-    const long long left = (long long)num_iters - next_cycle + 1;
-    const int64_t batch = left < 1 ? 1 : (left < cap ? left : cap);
-    for (int64_t i = 0; i < batch; ++i) {
-      if (use_graph) {
-        HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
-      } else {
-        const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
-        for (int id : ids) {
-          rc = launch(c, id, a);
-          if (rc) return rc;
-        }
-      }
-    }
-    HIP_TRY(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-    rc = finish(c, false);
-    if (rc) return rc;
-    if (!c->hsc->active) break;
-    next_cycle = c->hsc->cycle;
-    if (cap < 256) cap *= 2;
-  }
-  c->partials_valid = false;
-  if (cycles_run) *cycles_run = (int64_t)(c->hsc->cycle - cycle0);
-  const int err = c->hsc->err;
-  if (err != ERR_NONE) {
-    if (err >= 1 && err <= 9) return fail(c, err, kErrText[err]);
-    return fail(c, LULESH_ERR_HIP, "unknown device error code");
-  }
-  return 0;
+  std::vector<int64_t> cyc(c->n, 0);
+  int rc = run_cycles(c, num_iters, cyc.data());
+  if (cycles_run) *cycles_run = cyc[0];
+  return rc;
 }
 
-int lulesh_verify(lulesh_ctx *c, double out[4]) {
-  if (!c || !out) return LULESH_ERR_INVALID_ARG;
-  const int64_t nx = c->nx;
+int lulesh_batch_run(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
+  if (!c) return LULESH_ERR_INVALID_ARG;
+  return run_cycles(c, num_iters, cycles_run);
+}
+
+int lulesh_batch_verify(lulesh_ctx *c, int32_t inst, double out[4]) {
+  int rc = check_inst(c, inst);
+  if (rc) return rc;
+  if (!out) return LULESH_ERR_INVALID_ARG;
+  const int64_t nx = c->inst[inst].s;
   std::vector<double> e(nx * nx);
   HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpyAsync(e.data(), c->f.e, e.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  int rc = finish(c, false);
+  HIP_TRY(c, hipMemcpyAsync(e.data(), c->b.f.e + c->inst[inst].e0, e.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+  rc = finish(c, false);
   if (rc) return rc;
   // src/utils.jl:134-152
   double MaxAbsDiff = 0.0, MaxRelDiff = 0.0, TotalAbsDiff = 0.0;
@@ -548,6 +983,8 @@ int lulesh_verify(lulesh_ctx *c, double out[4]) {
   out[3] = MaxRelDiff;
   return 0;
 }
+
+int lulesh_verify(lulesh_ctx *c, double out[4]) { return lulesh_batch_verify(c, 0, out); }
 
 int lulesh_set_timing(lulesh_ctx *c, int32_t on) {
   if (!c) return LULESH_ERR_INVALID_ARG;
@@ -584,6 +1021,62 @@ int lulesh_math_check(int32_t op, const double *a, const double *b, double *out,
     rc = LULESH_ERR_HIP;
   (void)hipFree(d);
   return rc;
+}
+
+int lulesh_set_nodal_depth(lulesh_ctx *c, int32_t cz) {
+  if (!c || cz < 0 || cz > 1024) return LULESH_ERR_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  return plan_nodal(c, cz);
+}
+
+int32_t lulesh_nodal_depth(const lulesh_ctx *c) { return c ? c->inst[0].cz : -1; }
+
+int lulesh_autotune(lulesh_ctx *c, int32_t *chosen, double *best_ms) {
+  if (!c) return LULESH_ERR_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  // k_nodal alone reads the current node set and writes only the other one
+  // (the state becomes current only through k_flip), so it can be timed on
+  // the live state without changing it
+  const int cand[] = {2, 4, 8, 16, 32};
+  int best = -1;
+  float best_t = 0.f;
+  hipEvent_t ea = get_event(c), eb = get_event(c);
+  if (!ea || !eb) return fail(c, LULESH_ERR_HIP, "hipEventCreate failed");
+  int rc = 0;
+  for (int cz : cand) {
+    int maxs = 0;
+    for (const InstDesc &I : c->inst) maxs = I.s > maxs ? I.s : maxs;
+    if (cz > 2 && cz / 2 > maxs + 1) break;
+    rc = plan_nodal(c, cz);
+    if (rc) break;
+    const Batch a = make_args(c, 0, 0);
+    hipError_t e = hipSuccess;
+    for (int w = 0; w < 2 && e == hipSuccess; ++w) e = launch_kernel(KID_NODAL, c->stream, a);
+    if (e == hipSuccess) e = hipEventRecord(ea, c->stream);
+    for (int r = 0; r < 5 && e == hipSuccess; ++r) e = launch_kernel(KID_NODAL, c->stream, a);
+    if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(eb);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
+    if (e != hipSuccess) {
+      rc = hip_fail(c, e, "autotune");
+      break;
+    }
+    if (best < 0 || ms < best_t) {
+      best = cz;
+      best_t = ms;
+    }
+  }
+  c->event_pool.push_back(ea);
+  c->event_pool.push_back(eb);
+  if (rc) return rc;
+  rc = plan_nodal(c, best);
+  if (!rc) rc = clear_err(c);
+  if (!rc) rc = finish(c, false);
+  if (rc) return rc;
+  if (chosen) *chosen = best;
+  if (best_ms) *best_ms = best_t / 5.0;
+  return 0;
 }
 
 const char *lulesh_kernel_name(int32_t id) {

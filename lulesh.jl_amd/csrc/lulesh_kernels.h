@@ -1,5 +1,5 @@
-// Device-side data of one LULESH domain and the launcher interface between
-// the kernels and the C ABI.
+// Device-side data of a batch of LULESH domains and the launcher interface
+// between the kernels and the C ABI.  A single domain is a batch of one.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,7 +12,8 @@ constexpr int ERR_EARLY_VOLUME = 1, ERR_MID_VOLUME = 2, ERR_HG_VOLUME = 3, ERR_K
               ERR_LAG_VOLUME = 5, ERR_BC = 6, ERR_QSTOP = 7, ERR_EOS_VOLUME = 8,
               ERR_UPDATE_VOLUME = 9;
 
-// Domain scalars (src/types.jl:91-126) plus the device-side driver state.
+// Domain scalars (src/types.jl:91-126) plus the device-side driver state,
+// one per instance.
 struct DevScalars {
   double time, deltatime, dtcourant, dthydro;
   double stoptime, dtmax, dtfixed, deltatimemultlb, deltatimemultub;
@@ -20,7 +21,7 @@ struct DevScalars {
   double e_cut, p_cut, ss4o3, q_cut, v_cut, qlc_monoq, qqc_monoq, qqc;
   double eosvmax, eosvmin, pmin, emin, dvovmax, refdens;
   long long cycle;
-  long long num_iters;  // driver loop bound of the current lulesh_run
+  long long num_iters;  // driver loop bound of the current run
   int err;              // lowest error code raised (ERR_NONE when clean)
   int active;           // 1 while the current cycle executes (driver loop)
   int parity;           // which NodeSet holds the current x..zd
@@ -33,7 +34,8 @@ struct NodeSet {
   double *x, *y, *z, *xd, *yd, *zd;
 };
 
-// structure-of-arrays FP64 fields, one device buffer each
+// structure-of-arrays FP64 fields, one flat device buffer each; instance i
+// owns elements [e0, e0+E) and nodes [n0, n0+N) of every buffer
 struct DevFields {
   // element
   double *e, *p, *q, *ql, *qq, *v, *volo, *delv, *vdov, *arealg, *ss, *elemMass;
@@ -43,6 +45,37 @@ struct DevFields {
   // node
   NodeSet ns[2];
   double *xdd, *ydd, *zdd, *fx, *fy, *fz, *nodalMass;
+};
+
+// one instance of a (ragged) batch
+struct InstDesc {
+  int s;                 // elements per edge
+  int tiles;             // k_nodal tiles per edge
+  int64_t e0, n0;        // offsets into the flat arrays
+  int64_t E, N;          // sizes
+  int eb0, ebn;          // element-kernel blocks [eb0, eb0+ebn)
+  int nb0, nbn;          // node-kernel blocks
+  int gb0, gbn;          // k_nodal blocks
+  int cz;                // k_nodal node planes per chunk (the z-march depth)
+  double einit_scale;    // deposited energy = scale * the reference's einit
+  double hgcoef;         // initial hgcoef
+};
+
+struct Batch {
+  int n;                    // instances
+  const InstDesc *inst;     // device table [n]
+  const int *eblock_inst;   // instance of each element-kernel block
+  const int *nblock_inst;   // instance of each node-kernel block
+  const int *gblock_inst;   // instance of each k_nodal block
+  int eblocks, nblocks, gblocks;
+  DevScalars *sc;           // [n]
+  DevFields f;
+  double *dtpart;           // 2 per element-kernel block
+  int quirks;
+  int gated;                // kernels skip instances whose active flag is 0
+  int keep_aux;             // also write fx.., xdd.., dxx.., ql, qq
+  int loop_mode;            // k_dt evaluates the driver-loop test and the next dt
+  int64_t num_iters;
 };
 
 enum KernelId {
@@ -57,23 +90,19 @@ enum KernelId {
   KID_CLEAR_ERR = 8,
 };
 
-struct LaunchArgs {
-  DevScalars *sc;
-  DevFields f;
-  double *dtpart;  // 2 x qeos_blocks(E) time-constraint partial minima
-  int s;
-  int64_t E, N;
-  int quirks;
-  int gated;       // kernels exit when sc->active == 0
-  int keep_aux;    // also write fx.., xdd.., dxx.., ql, qq
-  int loop_mode;   // k_dt evaluates the driver-loop test and the next dt
-  int64_t num_iters;
-};
+constexpr int BLOCK = 256;
+// k_nodal tiling: 16x16 element columns around 15x15 node columns; each
+// workgroup marches `cz` node planes (chosen per context, see plan_nodal)
+constexpr int NODAL_NT = 15;
 
-hipError_t launch_init(hipStream_t st, int s, int64_t E, int64_t N, const DevFields &f,
-                       DevScalars *sc);
-hipError_t launch_kernel(int id, hipStream_t st, const LaunchArgs &a);
-int64_t qeos_blocks(int64_t E);
+inline int nodal_tiles(int s) { return (s + 1 + NODAL_NT - 1) / NODAL_NT; }
+inline int nodal_blocks(int s, int cz) {
+  const int t = nodal_tiles(s);
+  return t * t * ((s + 1 + cz - 1) / cz);
+}
+
+hipError_t launch_init(hipStream_t st, const Batch &b);
+hipError_t launch_kernel(int id, hipStream_t st, const Batch &b);
 hipError_t launch_math(hipStream_t st, int op, const double *a, const double *b, double *out,
                        int64_t n);
 

@@ -22,8 +22,6 @@
 
 namespace lulesh {
 
-constexpr int BLOCK = 256;
-
 // ---------------------------------------------------------------------------
 // structured-mesh indexing (buildMesh!, src/domain.jl:117-174)
 // ---------------------------------------------------------------------------
@@ -49,6 +47,58 @@ LDEV void gather8(const T *__restrict__ src, const int64_t (&nid)[8], T (&out)[8
 }
 
 LDEV void raise(DevScalars *sc, int code) { atomicMin(&sc->err, code); }
+
+// instance i's view of the flat field arrays
+LDEV DevFields slice(const DevFields &f, const InstDesc &I) {
+  DevFields g;
+  const int64_t a = I.e0, b = I.n0;
+  g.e = f.e + a; g.p = f.p + a; g.q = f.q + a; g.ql = f.ql + a; g.qq = f.qq + a;
+  g.v = f.v + a; g.volo = f.volo + a; g.delv = f.delv + a; g.vdov = f.vdov + a;
+  g.arealg = f.arealg + a; g.ss = f.ss + a; g.elemMass = f.elemMass + a; g.vnew = f.vnew + a;
+  g.delv_xi = f.delv_xi + a; g.delv_eta = f.delv_eta + a; g.delv_zeta = f.delv_zeta + a;
+  g.delx_xi = f.delx_xi + a; g.delx_eta = f.delx_eta + a; g.delx_zeta = f.delx_zeta + a;
+  g.dxx = f.dxx + a; g.dyy = f.dyy + a; g.dzz = f.dzz + a; g.elemBC = f.elemBC + a;
+  for (int k = 0; k < 2; ++k) {
+    g.ns[k].x = f.ns[k].x + b; g.ns[k].y = f.ns[k].y + b; g.ns[k].z = f.ns[k].z + b;
+    g.ns[k].xd = f.ns[k].xd + b; g.ns[k].yd = f.ns[k].yd + b; g.ns[k].zd = f.ns[k].zd + b;
+  }
+  g.xdd = f.xdd + b; g.ydd = f.ydd + b; g.zdd = f.zdd + b;
+  g.fx = f.fx + b; g.fy = f.fy + b; g.fz = f.fz + b; g.nodalMass = f.nodalMass + b;
+  return g;
+}
+
+// NodeSet k of a sliced field view, chosen without indexing a local array
+// by a run-time value (that would put the struct in scratch memory)
+LDEV NodeSet pick(const DevFields &f, int k) {
+  NodeSet r;
+  r.x = k ? f.ns[1].x : f.ns[0].x;
+  r.y = k ? f.ns[1].y : f.ns[0].y;
+  r.z = k ? f.ns[1].z : f.ns[0].z;
+  r.xd = k ? f.ns[1].xd : f.ns[0].xd;
+  r.yd = k ? f.ns[1].yd : f.ns[0].yd;
+  r.zd = k ? f.ns[1].zd : f.ns[0].zd;
+  return r;
+}
+
+// Per-block instance lookup for the element-, node- and k_nodal-shaped
+// grids: every block belongs to exactly one instance (instances start on a
+// block boundary), so the lookup is uniform over the block.
+#define LULESH_ELEM_PREAMBLE(B)                                                  \
+  const int inst_ = (B).eblock_inst[blockIdx.x];                               \
+  const InstDesc I = (B).inst[inst_];                                          \
+  DevScalars *sc = (B).sc + inst_;                                             \
+  const int s = I.s;                                                           \
+  const int64_t E = I.E;                                                       \
+  const int64_t e = (int64_t)(blockIdx.x - I.eb0) * BLOCK + threadIdx.x;       \
+  const DevFields f = slice((B).f, I);
+#define LULESH_NODE_PREAMBLE(B)                                                  \
+  const int inst_ = (B).nblock_inst[blockIdx.x];                               \
+  const InstDesc I = (B).inst[inst_];                                          \
+  DevScalars *sc = (B).sc + inst_;                                             \
+  const int s = I.s;                                                           \
+  const int64_t N = I.N;                                                       \
+  const int64_t n = (int64_t)(blockIdx.x - I.nb0) * BLOCK + threadIdx.x;       \
+  const DevFields f = slice((B).f, I);
 
 // timeIncrement!, src/domain.jl:920-961
 LDEV void time_increment(DevScalars *sc) {
@@ -77,8 +127,9 @@ LDEV void time_increment(DevScalars *sc) {
 // ---------------------------------------------------------------------------
 // Domain(prob) on the device (src/domain.jl:117-174, 615-771, 822-916)
 // ---------------------------------------------------------------------------
-__global__ void k_init_nodes(int s, int64_t N, DevFields f) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(BLOCK) void k_init_nodes(Batch B) {
+  LULESH_NODE_PREAMBLE(B)
+  (void)sc;
   if (n >= N) return;
   const int64_t s1 = s + 1;
   const int64_t col = n % s1, rest = n / s1;
@@ -93,8 +144,9 @@ __global__ void k_init_nodes(int s, int64_t N, DevFields f) {
   f.fx[n] = 0.0; f.fy[n] = 0.0; f.fz[n] = 0.0;
 }
 
-__global__ void k_init_elems(int s, int64_t E, DevFields f) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(BLOCK) void k_init_elems(Batch B) {
+  LULESH_ELEM_PREAMBLE(B)
+  (void)sc;
   if (e >= E) return;
   int64_t nid[8];
   elem_nodes(e, s, nid);
@@ -124,8 +176,9 @@ __global__ void k_init_elems(int s, int64_t E, DevFields f) {
 
 // nodal mass: volume/8 of each adjacent element added in ascending element
 // order, the order of the element loop at src/domain.jl:709-727
-__global__ void k_init_mass(int s, int64_t N, DevFields f) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(BLOCK) void k_init_mass(Batch B) {
+  LULESH_NODE_PREAMBLE(B)
+  (void)sc;
   if (n >= N) return;
   const int64_t s1 = s + 1;
   const int i = (int)(n % s1), j = (int)((n / s1) % s1), k = (int)(n / (s1 * s1));
@@ -148,8 +201,16 @@ __global__ void k_init_mass(int s, int64_t N, DevFields f) {
 }
 
 // scalars of src/domain.jl:663-701, energy deposit and dt0 (738-749)
-__global__ void k_init_scalars(int s, DevScalars *sc, double *e0, const double *volo) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// one thread per instance; the deposit is einit_scale * einit (1 for the
+// reference problem) and dt0 is computed from it
+__global__ void k_init_scalars(Batch B) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= B.n) return;
+  const InstDesc I = B.inst[i];
+  DevScalars *sc = B.sc + i;
+  const int s = I.s;
+  double *e0 = B.f.e + I.e0;
+  const double *volo = B.f.volo + I.e0;
   sc->dthydro = 1e20;
   sc->dtcourant = 1e20;
   sc->time = 0.;
@@ -164,7 +225,7 @@ __global__ void k_init_scalars(int s, DevScalars *sc, double *e0, const double *
   sc->q_cut = 1.0e-7;
   sc->u_cut = 1.0e-7;
   sc->v_cut = 1.0e-10;
-  sc->hgcoef = 3.0;
+  sc->hgcoef = I.hgcoef;
   sc->ss4o3 = 4.0 / 3.0;
   sc->qstop = 1.0e+12;
   sc->monoq_max_slope = 1.0;
@@ -180,7 +241,7 @@ __global__ void k_init_scalars(int s, DevScalars *sc, double *e0, const double *
   sc->refdens = 1.0;
   const double ebase = 3.948746e+7;
   const double scale = (double)s / 45.0;
-  const double einit = ((ebase * scale) * scale) * scale;
+  const double einit = (((ebase * scale) * scale) * scale) * I.einit_scale;
   e0[0] = einit;
   sc->deltatime = (.5 * cbrt_cr(volo[0])) / sqrt(2.0 * einit);
   sc->err = ERR_NONE;
@@ -192,15 +253,18 @@ __global__ void k_init_scalars(int s, DevScalars *sc, double *e0, const double *
 // ---------------------------------------------------------------------------
 // the cycle
 // ---------------------------------------------------------------------------
-__global__ void k_time_increment(DevScalars *sc) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) time_increment(sc);
+__global__ void k_time_increment(Batch B) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < B.n) time_increment(B.sc + i);
 }
 
 // driver loop head (examples/benchmark.jl:82-84): the first iteration is
 // entered whenever time < stoptime
-__global__ void k_run_begin(DevScalars *sc, int64_t num_iters) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  sc->num_iters = num_iters;
+__global__ void k_run_begin(Batch B) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= B.n) return;
+  DevScalars *sc = B.sc + i;
+  sc->num_iters = B.num_iters;
   sc->err = ERR_NONE;
   const int go = sc->time < sc->stoptime;
   sc->active = go;
@@ -210,7 +274,7 @@ __global__ void k_run_begin(DevScalars *sc, int64_t num_iters) {
 // ---------------------------------------------------------------------------
 // lagrangeNodal (src/domain.jl:1840-1872) as one kernel.
 //
-// A workgroup owns a column of NT x NT nodes and a chunk of CZ node planes.
+// A workgroup owns a column of NT x NT nodes and a chunk of cz node planes.
 // It marches up the chunk one element plane at a time: its 256 threads
 // compute the corner forces of the 16 x 16 elements around the node tile
 // (integrateStressForElems + calcFBHourglassForceForElems arithmetic), stage
@@ -224,24 +288,29 @@ __global__ void k_run_begin(DevScalars *sc, int64_t num_iters) {
 // position; src/domain.jl:1781-1838) into the other NodeSet, so tiles that
 // still read the old positions of shared halo nodes never see new values.
 // Elements on tile edges are computed by every tile that needs them
-// (16x16x(CZ+1) per 15x15xCZ nodes): no corner force ever goes to HBM.
+// (16x16x(cz+1) per 15x15xcz nodes): no corner force ever goes to HBM.
 // ---------------------------------------------------------------------------
 constexpr int ET = 16;  // element tile edge = threads per row
-constexpr int NT = 15;  // node tile edge
-constexpr int CZ = 16;  // node planes per chunk
+constexpr int NT = NODAL_NT;
 
-__global__ __launch_bounds__(BLOCK) void k_nodal(DevScalars *sc, int gated, int s, DevFields f,
-                                                 int quirks, int keep_aux, int tiles, int nblocks) {
-  if (gated && !sc->active) return;
+__global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
   // XCD-aware order: the dispatcher deals blocks round-robin over the 8 XCDs,
   // so give each XCD a contiguous run of tiles (their halos share its L2)
+  const int nblocks = B.gblocks;
   const int per = (nblocks + 7) / 8;
   const int vb = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
   if (vb >= nblocks) return;
-  const int tx = vb % tiles, ty = (vb / tiles) % tiles, chunk = vb / (tiles * tiles);
+  const int inst_ = B.gblock_inst[vb];
+  const InstDesc I = B.inst[inst_];
+  DevScalars *sc = B.sc + inst_;
+  if (B.gated && !sc->active) return;
+  const int s = I.s, tiles = I.tiles, quirks = B.quirks, keep_aux = B.keep_aux;
+  const DevFields f = slice(B.f, I);
+  const int lb = vb - I.gb0;
+  const int tx = lb % tiles, ty = (lb / tiles) % tiles, chunk = lb / (tiles * tiles);
   const int i0 = tx * NT, j0 = ty * NT;
-  const int k0 = chunk * CZ;
-  const int k1 = min(k0 + CZ, s + 1);
+  const int k0 = chunk * I.cz;
+  const int k1 = min(k0 + I.cz, s + 1);
   const int t = threadIdx.x, ti = t % ET, tj = t / ET;
   const int s1 = s + 1;
   // this thread's element in each plane, and its node column
@@ -259,7 +328,7 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(DevScalars *sc, int gated, int 
   // (1,1)->2, (0,1)->3, (1,0)->1, (0,0)->0 at the bottom; +4 at the top
 
   const int par = sc->parity;
-  const NodeSet in = f.ns[par], out = f.ns[par ^ 1];
+  const NodeSet in = pick(f, par), out = pick(f, par ^ 1);
   const double hourg = sc->hgcoef;
   const bool hg_on = hourg > 0.0;
   const double dt = sc->deltatime, u_cut = sc->u_cut;
@@ -441,29 +510,25 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(DevScalars *sc, int gated, int 
 }
 
 // the positions written by k_nodal become the current set
-__global__ void k_flip(DevScalars *sc, int gated) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (gated && !sc->active) return;
+__global__ void k_flip(Batch B) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= B.n) return;
+  DevScalars *sc = B.sc + i;
+  if (B.gated && !sc->active) return;
   sc->parity ^= 1;
-}
-
-void nodal_grid(int s, int &tiles, int &nblocks) {
-  tiles = (s + 1 + NT - 1) / NT;
-  const int chunks = (s + 1 + CZ - 1) / CZ;
-  nblocks = tiles * tiles * chunks;
 }
 
 // calcKinematicsForElems (src/domain.jl:2099-2147), the deviatoric loop
 // (2156-2171) and calcMonotonicQGradientsForElems (2175-2320)
-__global__ __launch_bounds__(BLOCK) void k_kin(DevScalars *sc, int gated, int s, int64_t E,
-                                               DevFields f, int keep_aux) {
-  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+__global__ __launch_bounds__(BLOCK) void k_kin(Batch B) {
+  LULESH_ELEM_PREAMBLE(B)
   if (e >= E) return;
-  if (gated && !sc->active) return;
+  if (B.gated && !sc->active) return;
+  const int keep_aux = B.keep_aux;
   int64_t nid[8];
   elem_nodes(e, s, nid);
   double X[8], Y[8], Z[8], XV[8], YV[8], ZV[8];
-  const NodeSet &S = f.ns[sc->parity];
+  const NodeSet S = pick(f, sc->parity);
   gather8(S.x, nid, X); gather8(S.y, nid, Y); gather8(S.z, nid, Z);
   gather8(S.xd, nid, XV); gather8(S.yd, nid, YV); gather8(S.zd, nid, ZV);
   const double volo = f.volo[e];
@@ -582,11 +647,11 @@ LDEV void block_min2(double &a, double &b) {
 // time; the reference's element-wise temporaries never mix elements),
 // updateVolumesForElems (2912-2930) and the per-element terms of
 // calcCourantConstraintForElems / calcHydroConstraintForElems (2964-3054)
-__global__ __launch_bounds__(BLOCK) void k_qeos(DevScalars *sc, int gated, int s, int64_t E,
-                                                DevFields f, double *__restrict__ dtpart, int quirks,
-                                                int keep_aux) {
-  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (gated && !sc->active) return;  // uniform over the grid
+__global__ __launch_bounds__(BLOCK) void k_qeos(Batch B) {
+  LULESH_ELEM_PREAMBLE(B)
+  if (B.gated && !sc->active) return;  // uniform over the block
+  const int quirks = B.quirks, keep_aux = B.keep_aux;
+  double *__restrict__ dtpart = B.dtpart;
   double cand_c = 1.0e+20, cand_h = 1.0e+20;
   if (e < E) {
     const double ptiny = 1e-36;
@@ -766,9 +831,10 @@ __global__ __launch_bounds__(BLOCK) void k_qeos(DevScalars *sc, int gated, int s
 
 // per-element time-constraint terms from the fields alone, for a
 // calcTimeConstraintsForElems call that does not follow k_qeos
-__global__ __launch_bounds__(BLOCK) void k_dtpart(DevScalars *sc, int64_t E, DevFields f,
-                                                  double *__restrict__ dtpart) {
-  const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+__global__ __launch_bounds__(BLOCK) void k_dtpart(Batch B) {
+  LULESH_ELEM_PREAMBLE(B)
+  (void)s;
+  double *__restrict__ dtpart = B.dtpart;
   double cand_c = 1.0e+20, cand_h = 1.0e+20;
   if (e < E) {
     const double vdov = f.vdov[e];
@@ -793,11 +859,14 @@ __global__ __launch_bounds__(BLOCK) void k_dtpart(DevScalars *sc, int64_t E, Dev
 
 // final minima; in driver-loop mode also the loop test of
 // examples/benchmark.jl:82,95 and the timeIncrement! of the next cycle
-__global__ __launch_bounds__(1024) void k_dt(DevScalars *sc, int gated, int64_t nblocks,
-                                             const double *__restrict__ dtpart, int loop_mode) {
-  if (gated && !sc->active) return;
+__global__ __launch_bounds__(1024) void k_dt(Batch B) {
+  const InstDesc I = B.inst[blockIdx.x];
+  DevScalars *sc = B.sc + blockIdx.x;
+  if (B.gated && !sc->active) return;
+  const int loop_mode = B.loop_mode;
+  const double *__restrict__ dtpart = B.dtpart + 2 * (int64_t)I.eb0;
   double a = 1.0e+20, b = 1.0e+20;
-  for (int64_t i = threadIdx.x; i < nblocks; i += blockDim.x) {
+  for (int64_t i = threadIdx.x; i < I.ebn; i += blockDim.x) {
     const double pa = dtpart[2 * i], pb = dtpart[2 * i + 1];
     a = pa < a ? pa : a;
     b = pb < b ? pb : b;
@@ -835,8 +904,9 @@ __global__ void k_math(int op, const double *__restrict__ a, const double *__res
   out[i] = op == 0 ? cbrt_cr(x) : (op == 1 ? sqrt(x) : x / b[i]);
 }
 
-__global__ void k_clear_err(DevScalars *sc) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) sc->err = ERR_NONE;
+__global__ void k_clear_err(Batch B) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < B.n) B.sc[i].err = ERR_NONE;
 }
 
 }  // namespace lulesh
@@ -847,16 +917,15 @@ __global__ void k_clear_err(DevScalars *sc) {
 namespace lulesh {
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+static inline unsigned per_inst_grid(int n) { return (unsigned)((n + 63) / 64); }
 
-hipError_t launch_init(hipStream_t st, int s, int64_t E, int64_t N, const DevFields &f, DevScalars *sc) {
-  hipLaunchKernelGGL(k_init_nodes, dim3(nblk(N, BLOCK)), dim3(BLOCK), 0, st, s, N, f);
-  hipLaunchKernelGGL(k_init_elems, dim3(nblk(E, BLOCK)), dim3(BLOCK), 0, st, s, E, f);
-  hipLaunchKernelGGL(k_init_mass, dim3(nblk(N, BLOCK)), dim3(BLOCK), 0, st, s, N, f);
-  hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, s, sc, f.e, f.volo);
+hipError_t launch_init(hipStream_t st, const Batch &b) {
+  hipLaunchKernelGGL(k_init_nodes, dim3(b.nblocks), dim3(BLOCK), 0, st, b);
+  hipLaunchKernelGGL(k_init_elems, dim3(b.eblocks), dim3(BLOCK), 0, st, b);
+  hipLaunchKernelGGL(k_init_mass, dim3(b.nblocks), dim3(BLOCK), 0, st, b);
+  hipLaunchKernelGGL(k_init_scalars, dim3(per_inst_grid(b.n)), dim3(64), 0, st, b);
   return hipGetLastError();
 }
-
-int64_t qeos_blocks(int64_t E) { return nblk(E, BLOCK); }
 
 hipError_t launch_math(hipStream_t st, int op, const double *a, const double *b, double *out,
                        int64_t n) {
@@ -864,43 +933,35 @@ hipError_t launch_math(hipStream_t st, int op, const double *a, const double *b,
   return hipGetLastError();
 }
 
-hipError_t launch_kernel(int id, hipStream_t st, const LaunchArgs &a) {
+hipError_t launch_kernel(int id, hipStream_t st, const Batch &b) {
   switch (id) {
-    case KID_NODAL: {
-      int tiles, nb;
-      nodal_grid(a.s, tiles, nb);
-      const int grid = ((nb + 7) / 8) * 8;
-      hipLaunchKernelGGL(k_nodal, dim3(grid), dim3(BLOCK), 0, st, a.sc, a.gated, a.s, a.f,
-                         a.quirks, a.keep_aux, tiles, nb);
+    case KID_NODAL:
+      // padded to a multiple of 8 for the XCD-aware block order
+      hipLaunchKernelGGL(k_nodal, dim3(((b.gblocks + 7) / 8) * 8), dim3(BLOCK), 0, st, b);
       break;
-    }
     case KID_FLIP:
-      hipLaunchKernelGGL(k_flip, dim3(1), dim3(64), 0, st, a.sc, a.gated);
+      hipLaunchKernelGGL(k_flip, dim3(per_inst_grid(b.n)), dim3(64), 0, st, b);
       break;
     case KID_KIN:
-      hipLaunchKernelGGL(k_kin, dim3(nblk(a.E, BLOCK)), dim3(BLOCK), 0, st, a.sc, a.gated, a.s,
-                         a.E, a.f, a.keep_aux);
+      hipLaunchKernelGGL(k_kin, dim3(b.eblocks), dim3(BLOCK), 0, st, b);
       break;
     case KID_QEOS:
-      hipLaunchKernelGGL(k_qeos, dim3(nblk(a.E, BLOCK)), dim3(BLOCK), 0, st, a.sc, a.gated, a.s,
-                         a.E, a.f, a.dtpart, a.quirks, a.keep_aux);
+      hipLaunchKernelGGL(k_qeos, dim3(b.eblocks), dim3(BLOCK), 0, st, b);
       break;
     case KID_DT:
-      hipLaunchKernelGGL(k_dt, dim3(1), dim3(1024), 0, st, a.sc, a.gated, qeos_blocks(a.E),
-                         a.dtpart, a.loop_mode);
+      hipLaunchKernelGGL(k_dt, dim3(b.n), dim3(1024), 0, st, b);
       break;
     case KID_DTPART:
-      hipLaunchKernelGGL(k_dtpart, dim3(nblk(a.E, BLOCK)), dim3(BLOCK), 0, st, a.sc, a.E, a.f,
-                         a.dtpart);
+      hipLaunchKernelGGL(k_dtpart, dim3(b.eblocks), dim3(BLOCK), 0, st, b);
       break;
     case KID_TIME_INCREMENT:
-      hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, st, a.sc);
+      hipLaunchKernelGGL(k_time_increment, dim3(per_inst_grid(b.n)), dim3(64), 0, st, b);
       break;
     case KID_RUN_BEGIN:
-      hipLaunchKernelGGL(k_run_begin, dim3(1), dim3(64), 0, st, a.sc, a.num_iters);
+      hipLaunchKernelGGL(k_run_begin, dim3(per_inst_grid(b.n)), dim3(64), 0, st, b);
       break;
     case KID_CLEAR_ERR:
-      hipLaunchKernelGGL(k_clear_err, dim3(1), dim3(64), 0, st, a.sc);
+      hipLaunchKernelGGL(k_clear_err, dim3(per_inst_grid(b.n)), dim3(64), 0, st, b);
       break;
     default:
       return hipErrorInvalidValue;

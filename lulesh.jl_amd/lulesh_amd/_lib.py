@@ -14,6 +14,7 @@ c_char_p, c_void_p = ctypes.c_char_p, ctypes.c_void_p
 P_double = ctypes.POINTER(c_double)
 P_int64 = ctypes.POINTER(c_int64)
 P_uint32 = ctypes.POINTER(ctypes.c_uint32)
+P_int32 = ctypes.POINTER(ctypes.c_int32)
 
 # name -> (restype, argtypes); mirrors include/lulesh_hip.h
 SIGNATURES = {
@@ -43,6 +44,25 @@ SIGNATURES = {
     "lulesh_kernel_time": (c_int, [c_void_p, c_int32, P_double, P_int64]),
     "lulesh_kernel_name": (c_char_p, [c_int32]),
     "lulesh_math_check": (c_int, [c_int32, P_double, P_double, P_double, c_int64, c_int32]),
+    "lulesh_batch_create": (c_int, [c_int32, P_int32, P_double, P_double, P_int32, c_int32,
+                                    c_int32, ctypes.POINTER(c_void_p)]),
+    "lulesh_batch_size": (c_int32, [c_void_p]),
+    "lulesh_batch_num_elem": (c_int64, [c_void_p, c_int32]),
+    "lulesh_batch_num_node": (c_int64, [c_void_p, c_int32]),
+    "lulesh_batch_download": (c_int, [c_void_p, c_int32, c_char_p, P_double, c_int64]),
+    "lulesh_batch_upload": (c_int, [c_void_p, c_int32, c_char_p, P_double, c_int64]),
+    "lulesh_batch_download_elem_bc": (c_int, [c_void_p, c_int32, P_uint32, c_int64]),
+    "lulesh_batch_upload_elem_bc": (c_int, [c_void_p, c_int32, P_uint32, c_int64]),
+    "lulesh_batch_get_scalar": (c_int, [c_void_p, c_int32, c_char_p, P_double]),
+    "lulesh_batch_set_scalar": (c_int, [c_void_p, c_int32, c_char_p, c_double]),
+    "lulesh_batch_run": (c_int, [c_void_p, c_int64, P_int64]),
+    "lulesh_batch_verify": (c_int, [c_void_p, c_int32, P_double]),
+    "lulesh_set_nodal_depth": (c_int, [c_void_p, c_int32]),
+    "lulesh_nodal_depth": (c_int32, [c_void_p]),
+    "lulesh_autotune": (c_int, [c_void_p, P_int32, P_double]),
+    "lulesh_grad_run": (c_int, [c_void_p, c_int64, c_int64, P_double, P_double, P_double,
+                                P_double]),
+    "lulesh_replay": (c_int, [c_void_p, P_double, c_int64]),
 }
 
 _lib = None

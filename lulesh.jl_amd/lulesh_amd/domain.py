@@ -74,7 +74,12 @@ def _check(lib, ctx, rc):
 
 
 class Domain:
-    """GPU-resident Domain{Float64}; Domain(prob) of src/domain.jl:467-771."""
+    """GPU-resident Domain{Float64}; Domain(prob) of src/domain.jl:467-771.
+
+    A Domain is also the per-instance view of a Batch (see Batch.instance);
+    field and scalar access then address that instance, while the
+    cycle-level functions (timeIncrement, lagrangeLeapFrog, run ...) always
+    advance every instance of the batch."""
 
     def __init__(self, prob, quirks=QUIRKS_REFERENCE, device=0):
         if not prob.structured:
@@ -89,6 +94,8 @@ class Domain:
         if rc != 0:
             raise LuleshError(rc, lib.lulesh_error_string(rc).decode())
         self._ctx = ctx
+        self._inst = 0
+        self._owner = True
         self.prob = prob
         self.quirks = int(quirks)
         self.sizeX = self.sizeY = self.sizeZ = prob.nx
@@ -97,11 +104,28 @@ class Domain:
         self.numReg, self.balance, self.cost = prob.nr, prob.balance, prob.cost
         self.comm = None
 
+    @classmethod
+    def _view(cls, batch, inst, nx):
+        d = object.__new__(cls)
+        object.__setattr__(d, "_lib", batch._lib)
+        object.__setattr__(d, "_ctx", batch._ctx)
+        object.__setattr__(d, "_inst", int(inst))
+        object.__setattr__(d, "_owner", False)
+        object.__setattr__(d, "_batch", batch)
+        object.__setattr__(d, "prob", None)
+        object.__setattr__(d, "quirks", batch.quirks)
+        for k in ("sizeX", "sizeY", "sizeZ"):
+            object.__setattr__(d, k, int(nx))
+        object.__setattr__(d, "numElem", int(batch._lib.lulesh_batch_num_elem(batch._ctx, inst)))
+        object.__setattr__(d, "numNode", int(batch._lib.lulesh_batch_num_node(batch._ctx, inst)))
+        object.__setattr__(d, "comm", None)
+        return d
+
     # -- lifetime --
     def close(self):
-        if getattr(self, "_ctx", None):
+        if getattr(self, "_ctx", None) and getattr(self, "_owner", False):
             self._lib.lulesh_destroy(self._ctx)
-            self._ctx = None
+        object.__setattr__(self, "_ctx", None)
 
     def __del__(self):
         try:
@@ -123,36 +147,36 @@ class Domain:
     def get(self, name):
         if name in SCALARS:
             v = ctypes.c_double()
-            _check(self._lib, self._ctx,
-                   self._lib.lulesh_get_scalar(self._ctx, name.encode(), ctypes.byref(v)))
+            _check(self._lib, self._ctx, self._lib.lulesh_batch_get_scalar(
+                self._ctx, self._inst, name.encode(), ctypes.byref(v)))
             return int(v.value) if name == "cycle" else v.value
         if name == "elemBC":
             out = np.empty(self.numElem, dtype=np.uint32)
-            _check(self._lib, self._ctx, self._lib.lulesh_download_elem_bc(
-                self._ctx, out.ctypes.data_as(_lib.P_uint32), out.size))
+            _check(self._lib, self._ctx, self._lib.lulesh_batch_download_elem_bc(
+                self._ctx, self._inst, out.ctypes.data_as(_lib.P_uint32), out.size))
             return out
         n = self._len(name)
         out = np.empty(n, dtype=np.float64)
-        _check(self._lib, self._ctx, self._lib.lulesh_download(
-            self._ctx, name.encode(), out.ctypes.data_as(_lib.P_double), n))
+        _check(self._lib, self._ctx, self._lib.lulesh_batch_download(
+            self._ctx, self._inst, name.encode(), out.ctypes.data_as(_lib.P_double), n))
         return out
 
     def set(self, name, value):
         if name in SCALARS:
-            _check(self._lib, self._ctx,
-                   self._lib.lulesh_set_scalar(self._ctx, name.encode(), float(value)))
+            _check(self._lib, self._ctx, self._lib.lulesh_batch_set_scalar(
+                self._ctx, self._inst, name.encode(), float(value)))
             return
         if name == "elemBC":
             a = np.ascontiguousarray(value, dtype=np.uint32)
-            _check(self._lib, self._ctx, self._lib.lulesh_upload_elem_bc(
-                self._ctx, a.ctypes.data_as(_lib.P_uint32), a.size))
+            _check(self._lib, self._ctx, self._lib.lulesh_batch_upload_elem_bc(
+                self._ctx, self._inst, a.ctypes.data_as(_lib.P_uint32), a.size))
             return
         a = np.ascontiguousarray(value, dtype=np.float64)
-        _check(self._lib, self._ctx, self._lib.lulesh_upload(
-            self._ctx, name.encode(), a.ctypes.data_as(_lib.P_double), a.size))
+        _check(self._lib, self._ctx, self._lib.lulesh_batch_upload(
+            self._ctx, self._inst, name.encode(), a.ctypes.data_as(_lib.P_double), a.size))
 
     def __getattr__(self, name):
-        if name.startswith("_") or name in ("prob",):
+        if name.startswith("_") or name in ("prob", "quirks"):
             raise AttributeError(name)
         if name in SCALARS or name in ELEM_FIELDS or name in NODE_FIELDS or name == "elemBC":
             return self.get(name)
@@ -219,8 +243,131 @@ def run(domain, num_iters):
 def verify(domain):
     """(e[1], MaxAbsDiff, TotalAbsDiff, MaxRelDiff) of src/utils.jl:131-152."""
     out = (ctypes.c_double * 4)()
-    _check(domain._lib, domain._ctx, domain._lib.lulesh_verify(domain._ctx, out))
+    _check(domain._lib, domain._ctx,
+           domain._lib.lulesh_batch_verify(domain._ctx, domain._inst, out))
     return tuple(out)
+
+
+def grad_run(domain, ncycles, checkpoint_every=20):
+    """Gradient of the total internal energy sum(e * elemMass) after
+    `ncycles` cycles with respect to the current e field and to hgcoef,
+    qlc_monoq and qqc_monoq, with the dt schedule of the forward pass held
+    fixed (replaces the Enzyme call of examples/benchmark.jl:85-86).  The
+    domain is left in the forward pass's end state."""
+    E = domain.numElem
+    ge = np.zeros(E)
+    gp = np.zeros(3)
+    loss = ctypes.c_double()
+    dt = np.zeros(int(ncycles))
+    _check(domain._lib, domain._ctx, domain._lib.lulesh_grad_run(
+        domain._ctx, int(ncycles), int(checkpoint_every), ge.ctypes.data_as(_lib.P_double),
+        gp.ctypes.data_as(_lib.P_double), ctypes.byref(loss), dt.ctypes.data_as(_lib.P_double)))
+    return {"loss": loss.value, "grad_e": ge, "grad_hgcoef": gp[0], "grad_qlc_monoq": gp[1],
+            "grad_qqc_monoq": gp[2], "dt": dt}
+
+
+def replay(domain, dt):
+    """Advance with a prescribed dt per cycle instead of timeIncrement!."""
+    a = np.ascontiguousarray(dt, dtype=np.float64)
+    _check(domain._lib, domain._ctx,
+           domain._lib.lulesh_replay(domain._ctx, a.ctypes.data_as(_lib.P_double), a.size))
+
+
+def total_internal_energy(domain):
+    e, m = domain.e, domain.elemMass
+    total = 0.0
+    for a, b in zip(e.tolist(), m.tolist()):
+        total += a * b
+    return total
+
+
+class Batch:
+    """Many independent Sedov domains advanced together (no reference
+    counterpart: the reference runs one Domain per process).
+
+    Instance i has nx[i] elements per edge, deposited energy
+    einit_scale[i] x the reference's einit (dt0 follows from it), initial
+    hgcoef[i] and region count nr[i].  All instances share one ragged SoA
+    layout; each cycle is one launch per kernel over every instance, with
+    per-instance dt, and an instance whose driver loop has ended is skipped."""
+
+    def __init__(self, nx, einit_scale=None, hgcoef=None, nr=None, quirks=QUIRKS_REFERENCE,
+                 device=0):
+        lib = _lib.load()
+        self._lib = lib
+        n = len(nx)
+        nx_a = (ctypes.c_int32 * n)(*[int(v) for v in nx])
+        sc_a = (ctypes.c_double * n)(*[float(v) for v in einit_scale]) if einit_scale is not None else None
+        hg_a = (ctypes.c_double * n)(*[float(v) for v in hgcoef]) if hgcoef is not None else None
+        nr_a = (ctypes.c_int32 * n)(*[int(v) for v in nr]) if nr is not None else None
+        ctx = ctypes.c_void_p()
+        rc = lib.lulesh_batch_create(n, nx_a, sc_a, hg_a, nr_a, int(quirks), int(device),
+                                     ctypes.byref(ctx))
+        if rc != 0:
+            raise LuleshError(rc, lib.lulesh_error_string(rc).decode())
+        self._ctx = ctx
+        self._inst = 0
+        self.quirks = int(quirks)
+        self.n = n
+        self.nx = [int(v) for v in nx]
+        self.views = [Domain._view(self, i, self.nx[i]) for i in range(n)]
+
+    def instance(self, i):
+        return self.views[i]
+
+    def __len__(self):
+        return self.n
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.lulesh_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, num_iters):
+        """Driver loop for every instance; returns cycles run per instance."""
+        out = (ctypes.c_int64 * self.n)()
+        _check(self._lib, self._ctx, self._lib.lulesh_batch_run(self._ctx, int(num_iters), out))
+        return np.array(out[:], dtype=np.int64)
+
+    def verify(self, i):
+        return verify(self.views[i])
+
+    def autotune(self):
+        """Pick the k_nodal z-march depth by timing it on the live state."""
+        ch, ms = ctypes.c_int32(), ctypes.c_double()
+        _check(self._lib, self._ctx,
+               self._lib.lulesh_autotune(self._ctx, ctypes.byref(ch), ctypes.byref(ms)))
+        return ch.value, ms.value
+
+    def set_nodal_depth(self, depth):
+        _check(self._lib, self._ctx, self._lib.lulesh_set_nodal_depth(self._ctx, int(depth)))
+
+    def set_timing(self, on=True):
+        _check(self._lib, self._ctx, self._lib.lulesh_set_timing(self._ctx, 1 if on else 0))
+
+    def keep_aux(self, on=True):
+        _check(self._lib, self._ctx, self._lib.lulesh_set_keep_aux(self._ctx, 1 if on else 0))
+
+
+def c5_instances(n=64, seed=0):
+    """The ragged Sedov workload: n instances drawn from numpy's PCG64
+    default_rng(seed), per instance in this order: s ~ U{16..64},
+    einit factor = exp(U[ln 0.5, ln 2]), hgcoef ~ U[1, 5],
+    nr ~ U{1, 5, 11, 16}."""
+    rng = np.random.default_rng(seed)
+    nx, scale, hg, nr = [], [], [], []
+    for _ in range(n):
+        nx.append(int(rng.integers(16, 65)))
+        scale.append(float(np.exp(rng.uniform(np.log(0.5), np.log(2.0)))))
+        hg.append(float(rng.uniform(1.0, 5.0)))
+        nr.append(int(rng.choice([1, 5, 11, 16])))
+    return nx, scale, hg, nr
 
 
 def verifyAndWriteFinalOutput(elapsed_time, domain, nx, numRanks=1, file=None):

@@ -890,57 +890,68 @@ int lagrange_kinematics(oracle_domain *d, double dt) {
 
 inline double sum4(double a, double b, double c, double e) { return ((a + b) + c) + e; }
 
+// src/domain.jl:2192-2317 for one element: X..ZV are its 8 nodes'
+// coordinates and velocities; out = {delx_xi, delx_eta, delx_zeta,
+// delv_xi, delv_eta, delv_zeta}
+void monoq_gradients_elem(const double *X, const double *Y, const double *Z, const double *XV,
+                          const double *YV, const double *ZV, double volo, double vnew,
+                          double *out) {
+  const double ptiny = 1.e-36;
+  const double vol = volo * vnew;
+  const double norm = 1.0 / (vol + ptiny);
+  double dxj = -0.25 * (sum4(X[0], X[1], X[5], X[4]) - sum4(X[3], X[2], X[6], X[7]));
+  double dyj = -0.25 * (sum4(Y[0], Y[1], Y[5], Y[4]) - sum4(Y[3], Y[2], Y[6], Y[7]));
+  double dzj = -0.25 * (sum4(Z[0], Z[1], Z[5], Z[4]) - sum4(Z[3], Z[2], Z[6], Z[7]));
+  double dxi = 0.25 * (sum4(X[1], X[2], X[6], X[5]) - sum4(X[0], X[3], X[7], X[4]));
+  double dyi = 0.25 * (sum4(Y[1], Y[2], Y[6], Y[5]) - sum4(Y[0], Y[3], Y[7], Y[4]));
+  double dzi = 0.25 * (sum4(Z[1], Z[2], Z[6], Z[5]) - sum4(Z[0], Z[3], Z[7], Z[4]));
+  double dxk = 0.25 * (sum4(X[4], X[5], X[6], X[7]) - sum4(X[0], X[1], X[2], X[3]));
+  double dyk = 0.25 * (sum4(Y[4], Y[5], Y[6], Y[7]) - sum4(Y[0], Y[1], Y[2], Y[3]));
+  double dzk = 0.25 * (sum4(Z[4], Z[5], Z[6], Z[7]) - sum4(Z[0], Z[1], Z[2], Z[3]));
+  double ax, ay, az, dxv, dyv, dzv;
+  // zeta: i cross j
+  ax = (dyi * dzj) - (dzi * dyj);
+  ay = (dzi * dxj) - (dxi * dzj);
+  az = (dxi * dyj) - (dyi * dxj);
+  out[2] = vol / std::sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  ax = ax * norm; ay = ay * norm; az = az * norm;
+  dxv = 0.25 * (sum4(XV[4], XV[5], XV[6], XV[7]) - sum4(XV[0], XV[1], XV[2], XV[3]));
+  dyv = 0.25 * (sum4(YV[4], YV[5], YV[6], YV[7]) - sum4(YV[0], YV[1], YV[2], YV[3]));
+  dzv = 0.25 * (sum4(ZV[4], ZV[5], ZV[6], ZV[7]) - sum4(ZV[0], ZV[1], ZV[2], ZV[3]));
+  out[5] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+  // xi: j cross k
+  ax = (dyj * dzk) - (dzj * dyk);
+  ay = (dzj * dxk) - (dxj * dzk);
+  az = (dxj * dyk) - (dyj * dxk);
+  out[0] = vol / std::sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  ax = ax * norm; ay = ay * norm; az = az * norm;
+  dxv = 0.25 * (sum4(XV[1], XV[2], XV[6], XV[5]) - sum4(XV[0], XV[3], XV[7], XV[4]));
+  dyv = 0.25 * (sum4(YV[1], YV[2], YV[6], YV[5]) - sum4(YV[0], YV[3], YV[7], YV[4]));
+  dzv = 0.25 * (sum4(ZV[1], ZV[2], ZV[6], ZV[5]) - sum4(ZV[0], ZV[3], ZV[7], ZV[4]));
+  out[3] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+  // eta: k cross i
+  ax = (dyk * dzi) - (dzk * dyi);
+  ay = (dzk * dxi) - (dxk * dzi);
+  az = (dxk * dyi) - (dyk * dxi);
+  out[1] = vol / std::sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  ax = ax * norm; ay = ay * norm; az = az * norm;
+  dxv = -0.25 * (sum4(XV[0], XV[1], XV[5], XV[4]) - sum4(XV[3], XV[2], XV[6], XV[7]));
+  dyv = -0.25 * (sum4(YV[0], YV[1], YV[5], YV[4]) - sum4(YV[3], YV[2], YV[6], YV[7]));
+  dzv = -0.25 * (sum4(ZV[0], ZV[1], ZV[5], ZV[4]) - sum4(ZV[3], ZV[2], ZV[6], ZV[7]));
+  out[4] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+}
+
 // src/domain.jl:2175-2320
 void monoq_gradients(oracle_domain *d) {
-  const double ptiny = 1.e-36;
   const int64_t E = d->numElem;
   OMP_FOR
   for (int64_t i = 0; i < E; ++i) {
-    double X[8], Y[8], Z[8], XV[8], YV[8], ZV[8];
+    double X[8], Y[8], Z[8], XV[8], YV[8], ZV[8], out[6];
     gather8(d, d->x, i, X); gather8(d, d->y, i, Y); gather8(d, d->z, i, Z);
     gather8(d, d->xd, i, XV); gather8(d, d->yd, i, YV); gather8(d, d->zd, i, ZV);
-    const double vol = d->volo[i] * d->vnew[i];
-    const double norm = 1.0 / (vol + ptiny);
-    double dxj = -0.25 * (sum4(X[0], X[1], X[5], X[4]) - sum4(X[3], X[2], X[6], X[7]));
-    double dyj = -0.25 * (sum4(Y[0], Y[1], Y[5], Y[4]) - sum4(Y[3], Y[2], Y[6], Y[7]));
-    double dzj = -0.25 * (sum4(Z[0], Z[1], Z[5], Z[4]) - sum4(Z[3], Z[2], Z[6], Z[7]));
-    double dxi = 0.25 * (sum4(X[1], X[2], X[6], X[5]) - sum4(X[0], X[3], X[7], X[4]));
-    double dyi = 0.25 * (sum4(Y[1], Y[2], Y[6], Y[5]) - sum4(Y[0], Y[3], Y[7], Y[4]));
-    double dzi = 0.25 * (sum4(Z[1], Z[2], Z[6], Z[5]) - sum4(Z[0], Z[3], Z[7], Z[4]));
-    double dxk = 0.25 * (sum4(X[4], X[5], X[6], X[7]) - sum4(X[0], X[1], X[2], X[3]));
-    double dyk = 0.25 * (sum4(Y[4], Y[5], Y[6], Y[7]) - sum4(Y[0], Y[1], Y[2], Y[3]));
-    double dzk = 0.25 * (sum4(Z[4], Z[5], Z[6], Z[7]) - sum4(Z[0], Z[1], Z[2], Z[3]));
-    double ax, ay, az, dxv, dyv, dzv;
-    // zeta: i cross j
-    ax = (dyi * dzj) - (dzi * dyj);
-    ay = (dzi * dxj) - (dxi * dzj);
-    az = (dxi * dyj) - (dyi * dxj);
-    d->delx_zeta[i] = vol / std::sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
-    ax = ax * norm; ay = ay * norm; az = az * norm;
-    dxv = 0.25 * (sum4(XV[4], XV[5], XV[6], XV[7]) - sum4(XV[0], XV[1], XV[2], XV[3]));
-    dyv = 0.25 * (sum4(YV[4], YV[5], YV[6], YV[7]) - sum4(YV[0], YV[1], YV[2], YV[3]));
-    dzv = 0.25 * (sum4(ZV[4], ZV[5], ZV[6], ZV[7]) - sum4(ZV[0], ZV[1], ZV[2], ZV[3]));
-    d->delv_zeta[i] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
-    // xi: j cross k
-    ax = (dyj * dzk) - (dzj * dyk);
-    ay = (dzj * dxk) - (dxj * dzk);
-    az = (dxj * dyk) - (dyj * dxk);
-    d->delx_xi[i] = vol / std::sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
-    ax = ax * norm; ay = ay * norm; az = az * norm;
-    dxv = 0.25 * (sum4(XV[1], XV[2], XV[6], XV[5]) - sum4(XV[0], XV[3], XV[7], XV[4]));
-    dyv = 0.25 * (sum4(YV[1], YV[2], YV[6], YV[5]) - sum4(YV[0], YV[3], YV[7], YV[4]));
-    dzv = 0.25 * (sum4(ZV[1], ZV[2], ZV[6], ZV[5]) - sum4(ZV[0], ZV[3], ZV[7], ZV[4]));
-    d->delv_xi[i] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
-    // eta: k cross i
-    ax = (dyk * dzi) - (dzk * dyi);
-    ay = (dzk * dxi) - (dxk * dzi);
-    az = (dxk * dyi) - (dyk * dxi);
-    d->delx_eta[i] = vol / std::sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
-    ax = ax * norm; ay = ay * norm; az = az * norm;
-    dxv = -0.25 * (sum4(XV[0], XV[1], XV[5], XV[4]) - sum4(XV[3], XV[2], XV[6], XV[7]));
-    dyv = -0.25 * (sum4(YV[0], YV[1], YV[5], YV[4]) - sum4(YV[3], YV[2], YV[6], YV[7]));
-    dzv = -0.25 * (sum4(ZV[0], ZV[1], ZV[5], ZV[4]) - sum4(ZV[3], ZV[2], ZV[6], ZV[7]));
-    d->delv_eta[i] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+    monoq_gradients_elem(X, Y, Z, XV, YV, ZV, d->volo[i], d->vnew[i], out);
+    d->delx_xi[i] = out[0]; d->delx_eta[i] = out[1]; d->delx_zeta[i] = out[2];
+    d->delv_xi[i] = out[3]; d->delv_eta[i] = out[4]; d->delv_zeta[i] = out[5];
   }
 }
 
@@ -956,6 +967,28 @@ inline double limit_phi(double delvm, double delvp, double norm, double mult, do
   if (phi < 0.0) phi = 0.0;
   if (phi > maxslope) phi = maxslope;
   return phi;
+}
+
+// src/domain.jl:2477-2511 for one element: q = {qlin, qquad}
+void monoq_q_elem(const double phi[3], const double dv[3], const double dx[3], double vdov,
+                  double elemMass, double volo, double vnew, double qlc, double qqc, double q[2]) {
+  if (vdov > 0.0) {
+    q[0] = 0.0;
+    q[1] = 0.0;
+    return;
+  }
+  double delvxxi = dv[0] * dx[0];
+  double delvxeta = dv[1] * dx[1];
+  double delvxzeta = dv[2] * dx[2];
+  if (delvxxi > 0.0) delvxxi = 0.0;
+  if (delvxeta > 0.0) delvxeta = 0.0;
+  if (delvxzeta > 0.0) delvxzeta = 0.0;
+  double rho = elemMass / (volo * vnew);
+  q[0] = ((-qlc) * rho) * (((delvxxi * (1.0 - phi[0])) + (delvxeta * (1.0 - phi[1]))) +
+                           (delvxzeta * (1.0 - phi[2])));
+  q[1] = (qqc * rho) * ((((delvxxi * delvxxi) * (1.0 - (phi[0] * phi[0]))) +
+                         ((delvxeta * delvxeta) * (1.0 - (phi[1] * phi[1])))) +
+                        ((delvxzeta * delvxzeta) * (1.0 - (phi[2] * phi[2]))));
 }
 
 // src/domain.jl:2322-2513 (+ 2516-2537)
@@ -1015,26 +1048,13 @@ int monoq_region(oracle_domain *d) {
 #pragma omp atomic write
       err = E_BC;
     }
-    double qlin, qquad;
-    if (d->vdov[i] > 0.0) {
-      qlin = 0.0;
-      qquad = 0.0;
-    } else {
-      double delvxxi = d->delv_xi[i] * d->delx_xi[i];
-      double delvxeta = d->delv_eta[i] * d->delx_eta[i];
-      double delvxzeta = d->delv_zeta[i] * d->delx_zeta[i];
-      if (delvxxi > 0.0) delvxxi = 0.0;
-      if (delvxeta > 0.0) delvxeta = 0.0;
-      if (delvxzeta > 0.0) delvxzeta = 0.0;
-      double rho = d->elemMass[i] / (d->volo[i] * d->vnew[i]);
-      qlin = ((-qlc) * rho) * (((delvxxi * (1.0 - phixi)) + (delvxeta * (1.0 - phieta))) +
-                               (delvxzeta * (1.0 - phizeta)));
-      qquad = (qqc * rho) * ((((delvxxi * delvxxi) * (1.0 - (phixi * phixi))) +
-                              ((delvxeta * delvxeta) * (1.0 - (phieta * phieta)))) +
-                             ((delvxzeta * delvxzeta) * (1.0 - (phizeta * phizeta))));
-    }
-    d->qq[i] = qquad;
-    d->ql[i] = qlin;
+    double q[2];
+    const double phi[3] = {phixi, phieta, phizeta};
+    const double dv[3] = {d->delv_xi[i], d->delv_eta[i], d->delv_zeta[i]};
+    const double dx[3] = {d->delx_xi[i], d->delx_eta[i], d->delx_zeta[i]};
+    monoq_q_elem(phi, dv, dx, d->vdov[i], d->elemMass[i], d->volo[i], d->vnew[i], qlc, qqc, q);
+    d->qq[i] = q[1];
+    d->ql[i] = q[0];
   }
   return err;
 }
@@ -1380,6 +1400,21 @@ extern "C" void oracle_calc_elem_fb_hourglass(const double *x, const double *y, 
     hgf[3 * n + 1] = fy[n];
     hgf[3 * n + 2] = fz[n];
   }
+}
+extern "C" void oracle_monoq_gradients_elem(const double *X, const double *Y, const double *Z,
+                                            const double *XV, const double *YV, const double *ZV,
+                                            double volo, double vnew, double *out) {
+  monoq_gradients_elem(X, Y, Z, XV, YV, ZV, volo, vnew, out);
+}
+// limiter + q of one element from its own and the selected neighbour delv
+extern "C" void oracle_monoq_region_elem(const double *dv, const double *dm, const double *dp,
+                                         const double *dx, double vdov, double elemMass,
+                                         double volo, double vnew, double qlc, double qqc,
+                                         double *q) {
+  const double ptiny = 1e-36;
+  double phi[3];
+  for (int k = 0; k < 3; ++k) phi[k] = limit_phi(dm[k], dp[k], 1.0 / (dv[k] + ptiny), 2.0, 1.0);
+  monoq_q_elem(phi, dv, dx, vdov, elemMass, volo, vnew, qlc, qqc, q);
 }
 extern "C" void oracle_eval_eos_elem(double vnew, double v_unused, double delv, double e_old,
                                      double p_old, double q_old, double ql, double qq,

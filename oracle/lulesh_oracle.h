@@ -70,6 +70,16 @@ void oracle_calc_elem_fb_hourglass(const double *x, const double *y, const doubl
                                    const double *xd, const double *yd, const double *zd,
                                    double determ, double ss, double mass, double hgcoef,
                                    int quirks, double *hgf);
+/* calcMonotonicQGradientsForElems of one element:
+ * out = {delx_xi, delx_eta, delx_zeta, delv_xi, delv_eta, delv_zeta} */
+void oracle_monoq_gradients_elem(const double *X, const double *Y, const double *Z,
+                                 const double *XV, const double *YV, const double *ZV,
+                                 double volo, double vnew, double *out);
+/* calcMonotonicQRegionForElems of one element (limiter mult 2, max slope 1):
+ * dv/dm/dp/dx[3] per direction xi, eta, zeta; q = {qlin, qquad} */
+void oracle_monoq_region_elem(const double *dv, const double *dm, const double *dp,
+                              const double *dx, double vdov, double elemMass, double volo,
+                              double vnew, double qlc, double qqc, double *q);
 /* one element of evalEOSForElems (+ calcSoundSpeed); out = {p, e, q, ss} */
 void oracle_eval_eos_elem(double vnew, double v_unused, double delv, double e_old,
                           double p_old, double q_old, double ql, double qq, double *out);

@@ -1,0 +1,181 @@
+"""Finite-difference checks of the hand-written adjoint element routines
+(lulesh.jl_amd/csrc/lulesh_adj.h, built here for the host) against central
+differences of the oracle's forward element functions."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+P = ctypes.POINTER(ctypes.c_double)
+UNIT = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                 [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=float)
+
+
+@pytest.fixture(scope="module")
+def adj(tmp_path_factory):
+    out = tmp_path_factory.mktemp("adj") / "libadj_host.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                    "-I", os.path.join(REPO, "lulesh.jl_amd", "csrc"),
+                    os.path.join(HERE, "adjoint_host", "adj_host.cpp"), "-o", str(out)],
+                   check=True)
+    L = ctypes.CDLL(str(out))
+    D = ctypes.c_double
+    L.adj_stress_rev.restype = D
+    L.adj_stress_rev.argtypes = [P, P, P, ctypes.c_int, D, P, P, P, P]
+    L.adj_hourglass_rev.argtypes = [P] * 6 + [D] * 4 + [ctypes.c_int] + [P] * 8
+    L.adj_monoq_grad_rev.restype = D
+    L.adj_monoq_grad_rev.argtypes = [P] * 6 + [D, D] + [P] * 7
+    return L
+
+
+def p(a):
+    return a.ctypes.data_as(P)
+
+
+def c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def fd(f, x0, h):
+    """central differences of scalar f over the entries of x0"""
+    g = np.zeros_like(x0)
+    for i in range(x0.size):
+        xp, xm = x0.copy(), x0.copy()
+        step = h * max(1.0, abs(x0.flat[i]))
+        xp.flat[i] += step
+        xm.flat[i] -= step
+        g.flat[i] = (f(xp) - f(xm)) / (2 * step)
+    return g
+
+
+@pytest.mark.parametrize("quirks", [0, 7])
+def test_stress_adjoint(adj, quirks):
+    rng = np.random.default_rng(1)
+    P0 = UNIT + 0.15 * rng.standard_normal((8, 3))
+    lF = rng.standard_normal((8, 3))
+    ms = 1.7
+
+    def loss(flat):
+        Q = flat.reshape(8, 3)
+        B = O.node_normals(c(Q[:, 0]), c(Q[:, 1]), c(Q[:, 2]), quirks)
+        return float((lF * ms * B).sum())
+
+    g_fd = fd(loss, P0.ravel().copy(), 1e-6).reshape(8, 3)
+    lX, lY, lZ = np.zeros(8), np.zeros(8), np.zeros(8)
+    lms = adj.adj_stress_rev(p(c(P0[:, 0])), p(c(P0[:, 1])), p(c(P0[:, 2])), quirks & 1, ms,
+                             p(c(lF)), p(lX), p(lY), p(lZ))
+    assert np.allclose(np.stack([lX, lY, lZ], 1), g_fd, rtol=1e-6, atol=1e-8)
+    B = O.node_normals(c(P0[:, 0]), c(P0[:, 1]), c(P0[:, 2]), quirks)
+    assert lms == pytest.approx(float((lF * B).sum()), rel=1e-12)
+
+
+@pytest.mark.parametrize("quirks", [0, 7])
+def test_hourglass_adjoint(adj, quirks):
+    rng = np.random.default_rng(2)
+    P0 = UNIT + 0.15 * rng.standard_normal((8, 3))
+    V0 = rng.standard_normal((8, 3))
+    lF = rng.standard_normal((8, 3))
+    determ, ss, mass, hg = 0.9, 1.3, 0.8, 3.0
+
+    def loss(flat):
+        Q, V = flat[:24].reshape(8, 3), flat[24:48].reshape(8, 3)
+        dt, s_, h_ = flat[48], flat[49], flat[50]
+        f = O.fb_hourglass(c(Q[:, 0]), c(Q[:, 1]), c(Q[:, 2]), c(V[:, 0]), c(V[:, 1]), c(V[:, 2]),
+                           dt, s_, mass, h_, quirks)
+        return float((lF * f).sum())
+
+    x0 = np.concatenate([P0.ravel(), V0.ravel(), [determ, ss, hg]])
+    g_fd = fd(loss, x0, 1e-6)
+    l = [np.zeros(8) for _ in range(6)]
+    sc = np.zeros(3)
+    adj.adj_hourglass_rev(p(c(P0[:, 0])), p(c(P0[:, 1])), p(c(P0[:, 2])), p(c(V0[:, 0])),
+                          p(c(V0[:, 1])), p(c(V0[:, 2])), determ, ss, mass, hg, quirks & 2,
+                          p(c(lF)), *[p(a) for a in l], p(sc))
+    g = np.concatenate([np.stack(l[:3], 1).ravel(), np.stack(l[3:], 1).ravel(), sc])
+    assert np.allclose(g, g_fd, rtol=1e-5, atol=1e-7)
+
+
+def test_volume_gradient(adj):
+    rng = np.random.default_rng(3)
+    P0 = UNIT + 0.2 * rng.standard_normal((8, 3))
+    g_fd = fd(lambda f: O.elem_volume(c(f.reshape(8, 3)[:, 0]), c(f.reshape(8, 3)[:, 1]),
+                                      c(f.reshape(8, 3)[:, 2])), P0.ravel().copy(), 1e-6)
+    gX, gY, gZ = np.zeros(8), np.zeros(8), np.zeros(8)
+    adj.adj_volume_grad(p(c(P0[:, 0])), p(c(P0[:, 1])), p(c(P0[:, 2])), p(gX), p(gY), p(gZ))
+    assert np.allclose(np.stack([gX, gY, gZ], 1).ravel(), g_fd, rtol=1e-7, atol=1e-9)
+
+
+def test_monoq_gradients_adjoint(adj):
+    rng = np.random.default_rng(4)
+    P0 = UNIT + 0.15 * rng.standard_normal((8, 3))
+    V0 = rng.standard_normal((8, 3))
+    lo = rng.standard_normal(6)
+    volo = 1.1
+
+    def loss(flat):
+        Q, V, vn = flat[:24].reshape(8, 3), flat[24:48].reshape(8, 3), flat[48]
+        out = np.zeros(6)
+        O.load().oracle_monoq_gradients_elem(p(c(Q[:, 0])), p(c(Q[:, 1])), p(c(Q[:, 2])),
+                                             p(c(V[:, 0])), p(c(V[:, 1])), p(c(V[:, 2])), volo,
+                                             vn, p(out))
+        return float((lo * out).sum())
+
+    O.load().oracle_monoq_gradients_elem.argtypes = [P] * 6 + [ctypes.c_double] * 2 + [P]
+    x0 = np.concatenate([P0.ravel(), V0.ravel(), [0.93]])
+    g_fd = fd(loss, x0, 1e-6)
+    l = [np.zeros(8) for _ in range(6)]
+    lvnew = adj.adj_monoq_grad_rev(p(c(P0[:, 0])), p(c(P0[:, 1])), p(c(P0[:, 2])), p(c(V0[:, 0])),
+                                   p(c(V0[:, 1])), p(c(V0[:, 2])), ctypes.c_double(volo),
+                                   ctypes.c_double(0.93), p(c(lo)), *[p(a) for a in l])
+    g = np.concatenate([np.stack(l[:3], 1).ravel(), np.stack(l[3:], 1).ravel(), [lvnew]])
+    assert np.allclose(g, g_fd, rtol=1e-6, atol=1e-8)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_monoq_region_adjoint(adj, seed):
+    rng = np.random.default_rng(100 + seed)
+    dv = rng.normal(-1.0, 0.5, 3)
+    x0 = np.concatenate([dv, dv * rng.uniform(0.3, 1.7, 3), dv * rng.uniform(0.3, 1.7, 3),
+                         rng.uniform(0.5, 1.5, 3), [-0.3, 1.2, 0.9, 0.95, 0.5, 2.0 / 3.0]])
+    lq = rng.standard_normal(2)
+    O.load().oracle_monoq_region_elem.argtypes = [P] * 4 + [ctypes.c_double] * 6 + [P]
+
+    def loss(f):
+        q = np.zeros(2)
+        O.load().oracle_monoq_region_elem(p(c(f[0:3])), p(c(f[3:6])), p(c(f[6:9])), p(c(f[9:12])),
+                                          *[float(v) for v in f[12:18]], p(q))
+        return float((lq * q).sum())
+
+    g_fd = fd(loss, x0, 1e-7)
+    out, lin = np.zeros(2), np.zeros(18)
+    adj.adj_monoq_region_rev(p(c(x0)), p(c(lq)), p(out), p(lin))
+    # vdov only selects the branch; elemMass and volo are constants of the run
+    keep = [i for i in range(18) if i not in (12, 13, 14)]
+    assert np.allclose(lin[keep], g_fd[keep], rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("case", ["compress", "expand", "cold"])
+def test_eos_adjoint(adj, case):
+    rng = np.random.default_rng(7)
+    if case == "compress":
+        x0 = np.array([0.97, -0.02, 3.0, 1.9, 0.05, 0.02, 0.01])
+    elif case == "expand":
+        x0 = np.array([1.03, 0.02, 3.0, 2.1, 0.0, 0.0, 0.0])
+    else:
+        x0 = np.array([0.99, -0.01, 0.5, 0.3, 0.01, 0.1, 0.2])
+    lo = rng.standard_normal(4)
+
+    def loss(f):
+        return float((lo * np.array(O.eos_elem(f[0], f[1], f[2], f[3], f[4], f[5], f[6]))).sum())
+
+    g_fd = fd(loss, x0, 1e-7)
+    out, li = np.zeros(4), np.zeros(7)
+    adj.adj_eos_rev(p(c(x0)), p(c(lo)), p(out), p(li))
+    assert np.allclose(out, O.eos_elem(*x0), rtol=1e-13)
+    assert np.allclose(li, g_fd, rtol=1e-5, atol=1e-8)

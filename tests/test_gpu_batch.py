@@ -1,0 +1,112 @@
+"""Ragged batches on the GPU: every instance of a batch must end bit-identical
+to the same domain run alone (and to the oracle), whatever its size, dt
+history or finishing cycle."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lulesh_amd import Batch, c5_instances, verify
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("e", "p", "q", "v", "ss", "x", "y", "z", "xd", "yd", "zd", "delv", "vdov", "arealg")
+SCAL = ("time", "deltatime", "cycle", "dtcourant", "dthydro", "hgcoef")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def same(da, db, tag):
+    for f in FIELDS:
+        a, b = da.get(f), db.get(f)
+        assert np.array_equal(bits(a), bits(b)), f"{tag}: field {f}"
+    for s in SCAL:
+        assert bits([da.get(s)])[0] == bits([db.get(s)])[0], f"{tag}: scalar {s}"
+
+
+def oracle_instance(nx, scale, hg, nthreads=4):
+    """Oracle domain with the batch's per-instance deposit and hgcoef: e[1] =
+    scale * einit and dt0 = (0.5*cbrt(volo[1]))/sqrt(2*e[1])."""
+    od = O.OracleDomain(nx, O.QUIRKS_REFERENCE, nthreads)
+    sc = nx / 45.0
+    einit = (((3.948746e+7 * sc) * sc) * sc) * scale
+    e = od.field("e")
+    e[0] = einit
+    od.set_field("e", e)
+    od.set("deltatime", (0.5 * O.cbrt(od.field("volo")[0])) / math.sqrt(2.0 * einit))
+    od.set("hgcoef", hg)
+    return od
+
+
+NX = [8, 12, 5, 16, 9]
+SCALE = [1.0, 0.7, 1.5, 1.2, 0.5]
+HG = [3.0, 1.5, 4.2, 2.0, 3.3]
+
+
+def test_batch_instances_equal_lone_runs_and_oracle():
+    print("[batch] 5 instances, 120 cycles", flush=True)
+    b = Batch(NX, SCALE, HG, [11, 1, 5, 16, 11])
+    cyc = b.run(120)
+    assert list(cyc) == [120] * 5
+    for i, nx in enumerate(NX):
+        lone = Batch([nx], [SCALE[i]], [HG[i]])
+        assert lone.run(120)[0] == 120
+        same(b.instance(i), lone.instance(0), f"instance {i} vs lone")
+        od = oracle_instance(nx, SCALE[i], HG[i])
+        assert od.run(120)[0] == 0
+        for f in ("e", "p", "q", "v", "ss", "x", "xd"):
+            assert np.array_equal(bits(b.instance(i).get(f)), bits(od.field(f))), (i, f)
+        lone.close()
+    b.close()
+
+
+def test_batch_instances_finish_at_their_own_cycle():
+    """Run to stop time: instances end at different cycles (188 and 352 for
+    the reference problem at s=8 and s=12); finished ones are skipped and
+    keep exactly the state a lone run ends with."""
+    print("[batch] run to stoptime", flush=True)
+    b = Batch([12, 8])
+    cyc = b.run(10 ** 9)
+    assert list(cyc) == [352, 188]
+    for i, nx in enumerate([12, 8]):
+        lone = Batch([nx])
+        lone.run(10 ** 9)
+        same(b.instance(i), lone.instance(0), f"to-stop instance {i}")
+        assert verify(b.instance(i)) == verify(lone.instance(0))
+        lone.close()
+    b.close()
+
+
+def test_c5_batch_matches_lone_runs():
+    nx, scale, hg, nr = c5_instances(64, seed=0)
+    print(f"[c5] 64 instances, {sum(s ** 3 for s in nx)} elements, 40 cycles", flush=True)
+    b = Batch(nx, scale, hg, nr)
+    assert list(b.run(40)) == [40] * 64
+    for i in (0, 7, 31, 63):
+        lone = Batch([nx[i]], [scale[i]], [hg[i]], [nr[i]])
+        lone.run(40)
+        same(b.instance(i), lone.instance(0), f"c5 instance {i}")
+        lone.close()
+    b.close()
+
+
+def test_autotune_leaves_state_unchanged_and_results_identical():
+    print("[autotune]", flush=True)
+    b = Batch(NX, SCALE, HG)
+    b.run(30)
+    before = {(i, f): b.instance(i).get(f) for i in range(len(NX)) for f in FIELDS}
+    depth, ms = b.autotune()
+    assert depth in (2, 4, 8, 16, 32) and ms > 0
+    for (i, f), a in before.items():
+        assert np.array_equal(bits(a), bits(b.instance(i).get(f))), (i, f)
+    b.run(60)
+    ref = Batch(NX, SCALE, HG)
+    ref.set_nodal_depth(2 if depth != 2 else 16)
+    ref.run(60)
+    for i in range(len(NX)):
+        same(b.instance(i), ref.instance(i), f"depth-independence {i}")
+    b.close()
+    ref.close()
