@@ -628,6 +628,13 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
     hipError_t e = launch_adjoint(AK_SEED, c->stream, adj_args(c, 0));
     if (e != hipSuccess) return hip_fail(c, e, "adjoint seed");
   }
+  // every k_flip toggles the node-set parity; the host follows it so the
+  // reverse sweep needs no synchronisation
+  int parity = 0;
+  rc = pull_scalars(c);
+  if (!rc) rc = finish(c, false);
+  if (rc) return rc;
+  parity = c->hsc[0].parity;
   const int64_t nseg = (ncycles + every - 1) / every;
   for (int64_t sg = nseg - 1; sg >= 0 && !rc; --sg) {
     const int64_t k0 = sg * every, k1 = (k0 + every < ncycles) ? k0 + every : ncycles;
@@ -639,6 +646,7 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
     for (int64_t k = k0; k < k1 && !rc; ++k) {
       rc = state_copy(c, B.seg + (k - k0) * state, 1);
       if (!rc) rc = replay_cycle(c, B.dt, cycle0, false);
+      parity ^= 1;
     }
     for (int64_t k = k1 - 1; k >= k0 && !rc; --k) {
       // S_k back in place, then the forward nodal + kinematics of cycle k
@@ -652,12 +660,8 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
       if (!rc) rc = launch(c, KID_FLIP, a);
       if (!rc) rc = launch(c, KID_KIN, a);
       if (rc) break;
-      // the parity after the flip is needed on the host to hand the kernels
-      // the right node sets
-      rc = pull_scalars(c);
-      if (!rc) rc = finish(c, false);
-      if (rc) break;
-      const AdjArgs aa = adj_args(c, c->hsc[0].parity);
+      parity ^= 1;
+      const AdjArgs aa = adj_args(c, parity);
       const int ks[] = {AK_QEOS, AK_KIN, AK_NODE1, AK_FORCE, AK_NODE2};
       for (int kk : ks) {
         hipError_t e = launch_adjoint(kk, c->stream, aa);

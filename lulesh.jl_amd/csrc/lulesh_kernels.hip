@@ -364,46 +364,60 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
       const double determ = f.volo[e] * v;
       if (v <= 0.0) raise(sc, ERR_HG_VOLUME);
       if (hg_on) {
-        double hourgam[8][4];
-        {
-          double dvdx[8], dvdy[8], dvdz[8];
-          volume_derivative(X, Y, Z, quirks, dvdx, dvdy, dvdz);
-          const double volinv = 1.0 / determ;
+        // Mode-major form of the same arithmetic: for each hourglass mode m
+        // the column hourgam[.][m], its three projections h_c[m] and the
+        // running sums sum_m hourgam[n][m] h_c[m] (m = 0..3 in order, as the
+        // reference's 4-term sums fold) -- fewer values live at once.
+        double dvdx[8], dvdy[8], dvdz[8];
+        volume_derivative(X, Y, Z, quirks, dvdx, dvdy, dvdz);
+        double hmx[4], hmy[4], hmz[4];
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            double hx = X[0] * gam(0, m), hy = Y[0] * gam(0, m), hz = Z[0] * gam(0, m);
+        for (int m = 0; m < 4; ++m) {
+          double hx = X[0] * gam(0, m), hy = Y[0] * gam(0, m), hz = Z[0] * gam(0, m);
 #pragma unroll
-            for (int n = 1; n < 8; ++n) {
-              hx = hx + X[n] * gam(n, m);
-              hy = hy + Y[n] * gam(n, m);
-              hz = hz + Z[n] * gam(n, m);
+          for (int n = 1; n < 8; ++n) {
+            hx = hx + X[n] * gam(n, m);
+            hy = hy + Y[n] * gam(n, m);
+            hz = hz + Z[n] * gam(n, m);
+          }
+          hmx[m] = hx; hmy[m] = hy; hmz[m] = hz;
+        }
+        const double volinv = 1.0 / determ;
+        double VX[8], VY[8], VZ[8];
+        gather8(in.xd, nid, VX); gather8(in.yd, nid, VY); gather8(in.zd, nid, VZ);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          double g[8];
+#pragma unroll
+          for (int n = 0; n < 8; ++n)
+            g[n] = gam(n, m) - volinv * (((dvdx[n] * hmx[m]) + (dvdy[n] * hmy[m])) + (dvdz[n] * hmz[m]));
+          double h0 = g[0] * VX[0], h1 = g[0] * VY[0], h2 = g[0] * VZ[0];
+#pragma unroll
+          for (int n = 1; n < 8; ++n) {
+            h0 = h0 + g[n] * VX[n];
+            h1 = h1 + g[n] * VY[n];
+            h2 = h2 + g[n] * VZ[n];
+          }
+#pragma unroll
+          for (int n = 0; n < 8; ++n) {
+            if (m == 0) {
+              hgf[n][0] = g[n] * h0;
+              hgf[n][1] = g[n] * h1;
+              hgf[n][2] = g[n] * h2;
+            } else {
+              hgf[n][0] = hgf[n][0] + g[n] * h0;
+              hgf[n][1] = hgf[n][1] + g[n] * h1;
+              hgf[n][2] = hgf[n][2] + g[n] * h2;
             }
-#pragma unroll
-            for (int n = 0; n < 8; ++n)
-              hourgam[n][m] =
-                  gam(n, m) - volinv * (((dvdx[n] * hx) + (dvdy[n] * hy)) + (dvdz[n] * hz));
           }
         }
         const double volume13 = cbrt_cr(determ);
         const double coefficient = ((((-hourg) * 0.01) * f.ss[e]) * f.elemMass[e]) / volume13;
-        const double *vel[3] = {in.xd, in.yd, in.zd};
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          double V[8];
-          gather8(vel[c], nid, V);
-          double h[4];
-#pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            double acc = hourgam[0][m] * V[0];
-#pragma unroll
-            for (int n = 1; n < 8; ++n) acc = acc + hourgam[n][m] * V[n];
-            h[m] = acc;
-          }
-#pragma unroll
-          for (int n = 0; n < 8; ++n)
-            hgf[n][c] = coefficient * ((((hourgam[n][0] * h[0]) + (hourgam[n][1] * h[1])) +
-                                        (hourgam[n][2] * h[2])) +
-                                       (hourgam[n][3] * h[3]));
+        for (int n = 0; n < 8; ++n) {
+          hgf[n][0] = coefficient * hgf[n][0];
+          hgf[n][1] = coefficient * hgf[n][1];
+          hgf[n][2] = coefficient * hgf[n][2];
         }
       }
     }

@@ -50,13 +50,15 @@ def test_batch_instances_equal_lone_runs_and_oracle():
     print("[batch] 5 instances, 120 cycles", flush=True)
     b = Batch(NX, SCALE, HG, [11, 1, 5, 16, 11])
     cyc = b.run(120)
-    assert list(cyc) == [120] * 5
+    # s=5 with 1.5x the deposit reaches the stop time at cycle 91 (oracle too)
+    assert list(cyc) == [120, 120, 91, 120, 120]
     for i, nx in enumerate(NX):
         lone = Batch([nx], [SCALE[i]], [HG[i]])
-        assert lone.run(120)[0] == 120
+        assert lone.run(120)[0] == cyc[i]
         same(b.instance(i), lone.instance(0), f"instance {i} vs lone")
         od = oracle_instance(nx, SCALE[i], HG[i])
         assert od.run(120)[0] == 0
+        assert od.get("cycle") == cyc[i]
         for f in ("e", "p", "q", "v", "ss", "x", "xd"):
             assert np.array_equal(bits(b.instance(i).get(f)), bits(od.field(f))), (i, f)
         lone.close()
