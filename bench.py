@@ -76,24 +76,29 @@ def bench_batched(n=64, seed=0, cap=5000):
         b.run(2)
         b.views[0].reset()
     t0 = time.perf_counter()
-    lone_cycles = [int(b.run(cap)[0]) for b in lone]
+    lone_cycles = [int(b.run(cap, check=False)[0]) for b in lone]
     seq = time.perf_counter() - t0
+    lone_err = [b.errors()[0] for b in lone]
     for b in lone:
         b.close()
     batch = Batch(nx, scale, hg, nr)
     batch.run(2)
-    batch.views[0].reset()
+    batch.views[0].reset()  # re-initialises every instance of the batch
     t0 = time.perf_counter()
-    cyc = batch.run(cap)
+    cyc = batch.run(cap, check=False)
     bt = time.perf_counter() - t0
+    errs = batch.errors()
     batch.close()
-    if [int(c) for c in cyc] != lone_cycles:
-        raise RuntimeError("batched cycle counts differ from the sequential runs")
+    if [int(c) for c in cyc] != lone_cycles or errs != lone_err:
+        raise RuntimeError("batched cycle counts or errors differ from the sequential runs")
     return {"config": f"{n} Sedov instances (seed {seed}), s in [{min(nx)}, {max(nx)}], "
-                      f"{sum(v ** 3 for v in nx)} elements, to stoptime or {cap} cycles",
+                      f"{sum(v ** 3 for v in nx)} elements, each to stoptime, {cap} cycles or "
+                      f"its first reference error",
             "sequential_ms": round(seq * 1e3, 2), "batched_ms": round(bt * 1e3, 2),
             "speedup": round(seq / bt, 3), "cycles_min": min(lone_cycles),
-            "cycles_max": max(lone_cycles), "reached_stoptime": int(sum(c < cap for c in lone_cycles))}
+            "cycles_max": max(lone_cycles), "zone_cycles": int(sum(c * v ** 3 for c, v in zip(lone_cycles, nx))),
+            "reached_stoptime": int(sum(c < cap and e == 0 for c, e in zip(lone_cycles, lone_err))),
+            "stopped_by_error": int(sum(e != 0 for e in lone_err))}
 
 
 def cpu_baseline(nx, cycles, nthreads):

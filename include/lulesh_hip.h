@@ -141,8 +141,14 @@ int lulesh_batch_download_elem_bc(lulesh_ctx *ctx, int32_t inst, uint32_t *dst, 
 int lulesh_batch_upload_elem_bc(lulesh_ctx *ctx, int32_t inst, const uint32_t *src, int64_t n);
 int lulesh_batch_get_scalar(lulesh_ctx *ctx, int32_t inst, const char *name, double *value);
 int lulesh_batch_set_scalar(lulesh_ctx *ctx, int32_t inst, const char *name, double value);
-/* the driver loop for every instance at once; cycles_run[i] per instance */
+/* the driver loop for every instance at once; cycles_run[i] per instance.
+ * An instance that raises one of the reference's errors stops there (its
+ * state is then undefined, as after the reference's exception) while the
+ * others run on; the call returns the lowest error code raised by any
+ * instance and lulesh_batch_error gives each instance's own code (0 if it
+ * ran cleanly) for the last run. */
 int lulesh_batch_run(lulesh_ctx *ctx, int64_t num_iters, int64_t *cycles_run);
+int lulesh_batch_error(const lulesh_ctx *ctx, int32_t inst);
 int lulesh_batch_verify(lulesh_ctx *ctx, int32_t inst, double out[4]);
 
 /* ---- differentiation ----------------------------------------------------
@@ -163,6 +169,22 @@ int lulesh_batch_verify(lulesh_ctx *ctx, int32_t inst, double out[4]);
 int lulesh_grad_run(lulesh_ctx *ctx, int64_t ncycles, int64_t checkpoint_every, double *grad_e,
                     double *grad_params, double *loss, double *dt_schedule);
 int lulesh_replay(lulesh_ctx *ctx, const double *dt, int64_t ncycles);
+/* lulesh_vjp_cycle is the reference's per-cycle call
+ * Enzyme.autodiff(lagrangeLeapFrog, Duplicated(domain, shadow))
+ * (examples/benchmark.jl:86): it advances the state by one lagrangeLeapFrog
+ * (at the current deltatime, as lulesh_lagrange_leapfrog does) and maps the
+ * context's shadow -- the adjoint of the state fields x y z xd yd zd e p q v
+ * ss -- in place from the adjoint of the state after the cycle to the
+ * adjoint of the state before it; grad_params[3] (may be NULL) accumulates
+ * (+=) d/d{hgcoef, qlc_monoq, qqc_monoq}.  deltatime, volo, elemMass and
+ * nodalMass are held fixed (no adjoint is returned for them) and the
+ * cycle's temporaries carry none.  The shadow starts at zero and is moved
+ * one field at a time by lulesh_shadow_upload/download (names as for
+ * lulesh_download, the eleven state fields only); lulesh_grad_run
+ * overwrites it.  One-domain contexts only. */
+int lulesh_vjp_cycle(lulesh_ctx *ctx, double *grad_params);
+int lulesh_shadow_upload(lulesh_ctx *ctx, const char *field, const double *src, int64_t n);
+int lulesh_shadow_download(lulesh_ctx *ctx, const char *field, double *dst, int64_t n);
 
 /* ---- measurement --------------------------------------------------------
  * With timing on, every kernel launch of the context is bracketed by HIP

@@ -17,9 +17,13 @@
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
 #define LHD __host__ __device__ inline
+// every loop here has constant bounds: unrolled, all array indices become
+// constants and the arrays live in registers
+#define LUNROLL _Pragma("unroll")
 #else
 #include <cmath>
 #define LHD inline
+#define LUNROLL
 #endif
 #include <stdint.h>
 
@@ -80,16 +84,19 @@ LHD int vd_slot(int call, bool q2) {
   return call;
 }
 
-LHD void volume_derivative(const double X[8], const double Y[8], const double Z[8], bool q2,
+template <bool Q2>
+LHD void volume_derivative_t(const double X[8], const double Y[8], const double Z[8],
                            double dvdx[8], double dvdy[8], double dvdz[8]) {
+  LUNROLL
   for (int c = 0; c < 8; ++c) {
     double x[6], y[6], z[6];
+    LUNROLL
     for (int k = 0; k < 6; ++k) {
       x[k] = X[vd_node(c, k)];
       y[k] = Y[vd_node(c, k)];
       z[k] = Z[vd_node(c, k)];
     }
-    const int sl = vd_slot(c, q2);
+    const int sl = vd_slot(c, Q2);
     dvdx[sl] = vd_P(y, z) / 12.0;
     dvdy[sl] = -vd_P(x, z) / 12.0;
     dvdz[sl] = -vd_P(y, x) / 12.0;
@@ -97,21 +104,25 @@ LHD void volume_derivative(const double X[8], const double Y[8], const double Z[
 }
 
 // adjoint of volume_derivative: adds to lX, lY, lZ
-LHD void volume_derivative_rev(const double X[8], const double Y[8], const double Z[8], bool q2,
+template <bool Q2>
+LHD void volume_derivative_rev_t(const double X[8], const double Y[8], const double Z[8],
                                const double ldx[8], const double ldy[8], const double ldz[8],
                                double lX[8], double lY[8], double lZ[8]) {
+  LUNROLL
   for (int c = 0; c < 8; ++c) {
     double x[6], y[6], z[6], lx[6] = {0, 0, 0, 0, 0, 0}, ly[6] = {0, 0, 0, 0, 0, 0},
                                  lz[6] = {0, 0, 0, 0, 0, 0};
+    LUNROLL
     for (int k = 0; k < 6; ++k) {
       x[k] = X[vd_node(c, k)];
       y[k] = Y[vd_node(c, k)];
       z[k] = Z[vd_node(c, k)];
     }
-    const int sl = vd_slot(c, q2);
+    const int sl = vd_slot(c, Q2);
     vd_P_rev(y, z, ldx[sl] / 12.0, ly, lz);
     vd_P_rev(x, z, -ldy[sl] / 12.0, lx, lz);
     vd_P_rev(y, x, -ldz[sl] / 12.0, ly, lx);
+    LUNROLL
     for (int k = 0; k < 6; ++k) {
       lX[vd_node(c, k)] += lx[k];
       lY[vd_node(c, k)] += ly[k];
@@ -120,11 +131,27 @@ LHD void volume_derivative_rev(const double X[8], const double Y[8], const doubl
   }
 }
 
+// the Q2 slot swap is a template parameter so that every array index is a
+// compile-time constant (a run-time slot would put the arrays in scratch)
+LHD void volume_derivative(const double X[8], const double Y[8], const double Z[8], bool q2,
+                           double dvdx[8], double dvdy[8], double dvdz[8]) {
+  if (q2) volume_derivative_t<true>(X, Y, Z, dvdx, dvdy, dvdz);
+  else volume_derivative_t<false>(X, Y, Z, dvdx, dvdy, dvdz);
+}
+
+LHD void volume_derivative_rev(const double X[8], const double Y[8], const double Z[8], bool q2,
+                               const double ldx[8], const double ldy[8], const double ldz[8],
+                               double lX[8], double lY[8], double lZ[8]) {
+  if (q2) volume_derivative_rev_t<true>(X, Y, Z, ldx, ldy, ldz, lX, lY, lZ);
+  else volume_derivative_rev_t<false>(X, Y, Z, ldx, ldy, ldz, lX, lY, lZ);
+}
+
 // exact gradient of calcElemVolume (= the quirk-free volume derivative)
 LHD void volume_grad_rev(const double X[8], const double Y[8], const double Z[8], double lvol,
                          double lX[8], double lY[8], double lZ[8]) {
   double dx[8], dy[8], dz[8];
   volume_derivative(X, Y, Z, false, dx, dy, dz);
+  LUNROLL
   for (int n = 0; n < 8; ++n) {
     lX[n] += lvol * dx[n];
     lY[n] += lvol * dy[n];
@@ -144,22 +171,26 @@ LHD int face_node(int f, int k) {
 
 LHD void face_normal(const double p[4][3], double a[3]) {
   double b0[3], b1[3];
+  LUNROLL
   for (int c = 0; c < 3; ++c) {
     b0[c] = 0.5 * (((p[3][c] + p[2][c]) - p[1][c]) - p[0][c]);
     b1[c] = 0.5 * (((p[2][c] + p[1][c]) - p[3][c]) - p[0][c]);
   }
   cross(b0, b1, a);
+  LUNROLL
   for (int c = 0; c < 3; ++c) a[c] *= 0.25;
 }
 
 LHD void face_normal_rev(const double p[4][3], const double la[3], double lp[4][3]) {
   double b0[3], b1[3], l0[3], l1[3];
+  LUNROLL
   for (int c = 0; c < 3; ++c) {
     b0[c] = 0.5 * (((p[3][c] + p[2][c]) - p[1][c]) - p[0][c]);
     b1[c] = 0.5 * (((p[2][c] + p[1][c]) - p[3][c]) - p[0][c]);
   }
   cross(b1, la, l0);  // d(la.(b0 x b1))/db0 = b1 x la
   cross(la, b0, l1);  // d/db1 = la x b0
+  LUNROLL
   for (int c = 0; c < 3; ++c) {
     const double g0 = 0.25 * l0[c], g1 = 0.25 * l1[c];
     lp[0][c] += 0.5 * (-g0 - g1);
@@ -171,9 +202,12 @@ LHD void face_normal_rev(const double p[4][3], const double la[3], double lp[4][
 
 LHD void node_normals(const double X[8], const double Y[8], const double Z[8], bool q1,
                       double B[8][3]) {
+  LUNROLL
   for (int n = 0; n < 8; ++n) B[n][0] = B[n][1] = B[n][2] = 0.0;
+  LUNROLL
   for (int f = 0; f < 6; ++f) {
     double p[4][3], a[3];
+    LUNROLL
     for (int k = 0; k < 4; ++k) {
       const int nn = face_node(f, k);
       p[k][0] = X[nn];
@@ -181,8 +215,10 @@ LHD void node_normals(const double X[8], const double Y[8], const double Z[8], b
       p[k][2] = Z[nn];
     }
     face_normal(p, a);
+    LUNROLL
     for (int k = 0; k < 4; ++k) {
       const int nn = face_node(f, k);
+      LUNROLL
       for (int c = 0; c < 3; ++c) B[nn][c] += a[c];
     }
     if (f == 4 && q1) {  // node 1: areaY once more, areaZ not at all
@@ -192,32 +228,43 @@ LHD void node_normals(const double X[8], const double Y[8], const double Z[8], b
   }
 }
 
-// lF[n][c]: adjoint of the stress corner force.  Adds to lX.., returns lms.
-LHD double stress_rev(const double X[8], const double Y[8], const double Z[8], bool q1, double ms,
-                      const double lF[8][3], double lX[8], double lY[8], double lZ[8]) {
-  double B[8][3];
-  node_normals(X, Y, Z, q1, B);
+struct HgArraySrcF {
+  const double (*lF)[3];
+  LHD double lf(int n, int c) const { return lF[n][c]; }
+};
+
+// lF[n][c]: adjoint of the stress corner force f[n][c] = ms * B[n][c].
+// Face f adds its normal a_f to B of its four nodes (and, quirk Q1, face 5
+// adds a_y once more and subtracts a_z at node 1), so with S_f the sum of
+// lF over the nodes a_f reaches, lms = sum_f a_f . S_f and the adjoint of
+// a_f is ms * S_f.  Reads lF through src.lf(n, c); adds to lX..; returns lms.
+template <class Src>
+LHD double stress_rev_s(const double X[8], const double Y[8], const double Z[8], bool q1, double ms,
+                        const Src &src, double lX[8], double lY[8], double lZ[8]) {
   double lms = 0.0;
-  for (int n = 0; n < 8; ++n)
-    for (int c = 0; c < 3; ++c) lms += lF[n][c] * B[n][c];
+  LUNROLL
   for (int f = 0; f < 6; ++f) {
-    double la[3] = {0.0, 0.0, 0.0};
+    double S[3] = {0.0, 0.0, 0.0}, p[4][3];
+    LUNROLL
     for (int k = 0; k < 4; ++k) {
       const int nn = face_node(f, k);
-      for (int c = 0; c < 3; ++c) la[c] += ms * lF[nn][c];
-    }
-    if (f == 4 && q1) {
-      la[1] += ms * lF[0][1];
-      la[2] -= ms * lF[0][2];
-    }
-    double p[4][3], lp[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int k = 0; k < 4; ++k) {
-      const int nn = face_node(f, k);
+      LUNROLL
+      for (int c = 0; c < 3; ++c) S[c] += src.lf(nn, c);
       p[k][0] = X[nn];
       p[k][1] = Y[nn];
       p[k][2] = Z[nn];
     }
+    if (f == 4 && q1) {
+      S[1] += src.lf(0, 1);
+      S[2] -= src.lf(0, 2);
+    }
+    double a[3];
+    face_normal(p, a);
+    lms += a[0] * S[0] + a[1] * S[1] + a[2] * S[2];
+    const double la[3] = {ms * S[0], ms * S[1], ms * S[2]};
+    double lp[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     face_normal_rev(p, la, lp);
+    LUNROLL
     for (int k = 0; k < 4; ++k) {
       const int nn = face_node(f, k);
       lX[nn] += lp[k][0];
@@ -228,6 +275,12 @@ LHD double stress_rev(const double X[8], const double Y[8], const double Z[8], b
   return lms;
 }
 
+LHD double stress_rev(const double X[8], const double Y[8], const double Z[8], bool q1, double ms,
+                      const double lF[8][3], double lX[8], double lY[8], double lZ[8]) {
+  HgArraySrcF src{lF};
+  return stress_rev_s(X, Y, Z, q1, ms, src, lX, lY, lZ);
+}
+
 // ---------------------------------------------------------------------------
 // FB hourglass corner forces (src/domain.jl:1505-1639), adjoint with respect
 // to coordinates, velocities, determ (= volo*v), ss and hgcoef
@@ -236,84 +289,148 @@ struct HgRev {
   double ldeterm, lss, lhgcoef;
 };
 
-LHD HgRev hourglass_rev(const double X[8], const double Y[8], const double Z[8],
-                        const double VX[8], const double VY[8], const double VZ[8], double determ,
-                        double ss, double mass, double hourg, double cbrt_determ, bool q2,
-                        const double lF[8][3], double lX[8], double lY[8], double lZ[8],
-                        double lVX[8], double lVY[8], double lVZ[8]) {
+// reverse of voluDer call c (the derivative landing in slot vd_slot(c)),
+// scaled by the slot's adjoint (lx, ly, lz); adds to lX, lY, lZ
+LHD void volume_derivative_call_rev(const double X[8], const double Y[8], const double Z[8], int c,
+                                    double lx, double ly, double lz, double lX[8], double lY[8],
+                                    double lZ[8]) {
+  double x[6], y[6], z[6], gx[6] = {0, 0, 0, 0, 0, 0}, gy[6] = {0, 0, 0, 0, 0, 0},
+                           gz[6] = {0, 0, 0, 0, 0, 0};
+  LUNROLL
+  for (int k = 0; k < 6; ++k) {
+    x[k] = X[vd_node(c, k)];
+    y[k] = Y[vd_node(c, k)];
+    z[k] = Z[vd_node(c, k)];
+  }
+  vd_P_rev(y, z, lx / 12.0, gy, gz);
+  vd_P_rev(x, z, -ly / 12.0, gx, gz);
+  vd_P_rev(y, x, -lz / 12.0, gy, gx);
+  LUNROLL
+  for (int k = 0; k < 6; ++k) {
+    lX[vd_node(c, k)] += gx[k];
+    lY[vd_node(c, k)] += gy[k];
+    lZ[vd_node(c, k)] += gz[k];
+  }
+}
+
+// Streaming form.  With P_n the node positions, d_n the volume derivative,
+// h_m = sum_n gamma[n][m] P_n and g[n][m] = gamma[n][m] - (d_n . h_m)/determ,
+// the forward force is F[n][c] = coef * sum_m g[n][m] H[c][m] with
+// H[c][m] = sum_n g[n][m] V[c][n].  With A[m][c] = sum_n g[n][m] lF[n][c]:
+//   lcoef = sum A[m][c] H[c][m],  lV[c][n] = coef sum_m A[m][c] g[n][m],
+//   lg[n][m] = coef sum_c (lF[n][c] H[c][m] + A[m][c] V[c][n]),
+// and lg flows to determ, d_n and h_m.  Two node-major passes (the first
+// forms H and A, the second lg and everything downstream) keep few values
+// live: velocities and force adjoints are read node by node from `src`
+// (src.v(c, n), src.lf(n, c)) and each lV[c][n] is final when formed, so it
+// goes straight to `sink.lv(c, n, value)`.  Adds the position adjoints to
+// lX, lY, lZ.
+template <bool Q2, class Src, class Sink>
+LHD HgRev hourglass_rev_s(const double X[8], const double Y[8], const double Z[8], const Src &src,
+                          Sink &sink, double determ, double ss, double mass, double hourg,
+                          double cbrt_determ, double lX[8], double lY[8], double lZ[8]) {
   double dvdx[8], dvdy[8], dvdz[8];
-  volume_derivative(X, Y, Z, q2, dvdx, dvdy, dvdz);
+  volume_derivative_t<Q2>(X, Y, Z, dvdx, dvdy, dvdz);
   const double volinv = 1.0 / determ;
-  double hx[4], hy[4], hz[4], hg[8][4];
+  double hx[4], hy[4], hz[4];
+  LUNROLL
   for (int m = 0; m < 4; ++m) {
     hx[m] = hy[m] = hz[m] = 0.0;
+    LUNROLL
     for (int n = 0; n < 8; ++n) {
       hx[m] += X[n] * gamma_of(n, m);
       hy[m] += Y[n] * gamma_of(n, m);
       hz[m] += Z[n] * gamma_of(n, m);
     }
-    for (int n = 0; n < 8; ++n)
-      hg[n][m] = gamma_of(n, m) - volinv * (dvdx[n] * hx[m] + dvdy[n] * hy[m] + dvdz[n] * hz[m]);
+  }
+  // pass 1: H and A
+  double H[3][4], A[4][3];
+  LUNROLL
+  for (int m = 0; m < 4; ++m)
+    LUNROLL
+    for (int c = 0; c < 3; ++c) H[c][m] = A[m][c] = 0.0;
+  LUNROLL
+  for (int n = 0; n < 8; ++n) {
+    const double v0 = src.v(0, n), v1 = src.v(1, n), v2 = src.v(2, n);
+    const double f0 = src.lf(n, 0), f1 = src.lf(n, 1), f2 = src.lf(n, 2);
+    LUNROLL
+    for (int m = 0; m < 4; ++m) {
+      const double g = gamma_of(n, m) - volinv * (dvdx[n] * hx[m] + dvdy[n] * hy[m] + dvdz[n] * hz[m]);
+      H[0][m] += g * v0; H[1][m] += g * v1; H[2][m] += g * v2;
+      A[m][0] += g * f0; A[m][1] += g * f1; A[m][2] += g * f2;
+    }
   }
   const double K = -hourg * 0.01 * mass / cbrt_determ;  // coef = K * ss
   const double coef = K * ss;
-  const double *V[3] = {VX, VY, VZ};
-  double *lV[3] = {lVX, lVY, lVZ};
-  double h[3][4];
-  for (int c = 0; c < 3; ++c)
-    for (int m = 0; m < 4; ++m) {
-      h[c][m] = 0.0;
-      for (int n = 0; n < 8; ++n) h[c][m] += hg[n][m] * V[c][n];
-    }
-  double lcoef = 0.0, lhg[8][4], lh[3][4];
-  for (int n = 0; n < 8; ++n)
-    for (int m = 0; m < 4; ++m) lhg[n][m] = 0.0;
-  for (int c = 0; c < 3; ++c)
-    for (int m = 0; m < 4; ++m) lh[c][m] = 0.0;
-  for (int n = 0; n < 8; ++n)
-    for (int c = 0; c < 3; ++c) {
-      double g = 0.0;
-      for (int m = 0; m < 4; ++m) g += hg[n][m] * h[c][m];
-      lcoef += lF[n][c] * g;
-      for (int m = 0; m < 4; ++m) {
-        lhg[n][m] += coef * lF[n][c] * h[c][m];
-        lh[c][m] += coef * lF[n][c] * hg[n][m];
-      }
-    }
-  for (int c = 0; c < 3; ++c)
-    for (int m = 0; m < 4; ++m)
-      for (int n = 0; n < 8; ++n) {
-        lhg[n][m] += lh[c][m] * V[c][n];
-        lV[c][n] += lh[c][m] * hg[n][m];
-      }
-  double lvolinv = 0.0, ldx[8], ldy[8], ldz[8], lhx[4] = {0, 0, 0, 0}, lhy[4] = {0, 0, 0, 0},
-                        lhz[4] = {0, 0, 0, 0};
+  double lcoef = 0.0;
+  LUNROLL
+  for (int m = 0; m < 4; ++m)
+    LUNROLL
+    for (int c = 0; c < 3; ++c) lcoef += A[m][c] * H[c][m];
+  // pass 2: lg node by node -> lV, determ, d_n (reversed at once), h_m
+  double lvolinv = 0.0, lhx[4] = {0, 0, 0, 0}, lhy[4] = {0, 0, 0, 0}, lhz[4] = {0, 0, 0, 0};
+  LUNROLL
   for (int n = 0; n < 8; ++n) {
-    ldx[n] = ldy[n] = ldz[n] = 0.0;
+    const double v0 = src.v(0, n), v1 = src.v(1, n), v2 = src.v(2, n);
+    const double f0 = src.lf(n, 0), f1 = src.lf(n, 1), f2 = src.lf(n, 2);
+    double lv0 = 0.0, lv1 = 0.0, lv2 = 0.0, ldx = 0.0, ldy = 0.0, ldz = 0.0;
+    LUNROLL
     for (int m = 0; m < 4; ++m) {
-      const double L = lhg[n][m];
-      lvolinv -= L * (dvdx[n] * hx[m] + dvdy[n] * hy[m] + dvdz[n] * hz[m]);
-      ldx[n] -= volinv * L * hx[m];
-      ldy[n] -= volinv * L * hy[m];
-      ldz[n] -= volinv * L * hz[m];
-      lhx[m] -= volinv * L * dvdx[n];
-      lhy[m] -= volinv * L * dvdy[n];
-      lhz[m] -= volinv * L * dvdz[n];
+      const double dh = dvdx[n] * hx[m] + dvdy[n] * hy[m] + dvdz[n] * hz[m];
+      const double g = gamma_of(n, m) - volinv * dh;
+      lv0 += A[m][0] * g; lv1 += A[m][1] * g; lv2 += A[m][2] * g;
+      const double lg = coef * (f0 * H[0][m] + f1 * H[1][m] + f2 * H[2][m] +
+                                A[m][0] * v0 + A[m][1] * v1 + A[m][2] * v2);
+      lvolinv -= lg * dh;
+      const double s = volinv * lg;
+      ldx -= s * hx[m]; ldy -= s * hy[m]; ldz -= s * hz[m];
+      lhx[m] -= s * dvdx[n]; lhy[m] -= s * dvdy[n]; lhz[m] -= s * dvdz[n];
     }
+    sink.lv(0, n, coef * lv0);
+    sink.lv(1, n, coef * lv1);
+    sink.lv(2, n, coef * lv2);
+    // slot n holds voluDer call n, except calls 1 and 3 swap slots without Q2
+    const int call = (!Q2 && n == 1) ? 3 : ((!Q2 && n == 3) ? 1 : n);
+    volume_derivative_call_rev(X, Y, Z, call, ldx, ldy, ldz, lX, lY, lZ);
   }
+  LUNROLL
   for (int n = 0; n < 8; ++n)
+    LUNROLL
     for (int m = 0; m < 4; ++m) {
       lX[n] += lhx[m] * gamma_of(n, m);
       lY[n] += lhy[m] * gamma_of(n, m);
       lZ[n] += lhz[m] * gamma_of(n, m);
     }
-  volume_derivative_rev(X, Y, Z, q2, ldx, ldy, ldz, lX, lY, lZ);
   HgRev r;
   // coef = -hourg*0.01*ss*mass * determ^(-1/3)
   r.ldeterm = -lvolinv * volinv * volinv - lcoef * coef / (3.0 * determ);
   r.lss = lcoef * K;
   r.lhgcoef = lcoef * (-0.01 * ss * mass / cbrt_determ);
   return r;
+}
+
+struct HgArraySrc {
+  const double *V[3];
+  const double (*lF)[3];
+  LHD double v(int c, int n) const { return V[c][n]; }
+  LHD double lf(int n, int c) const { return lF[n][c]; }
+};
+struct HgArraySink {
+  double *lV[3];
+  LHD void lv(int c, int n, double x) { lV[c][n] += x; }
+};
+
+// array form (host tests): adds to lX.. and lVX..
+LHD HgRev hourglass_rev(const double X[8], const double Y[8], const double Z[8],
+                        const double VX[8], const double VY[8], const double VZ[8], double determ,
+                        double ss, double mass, double hourg, double cbrt_determ, bool q2,
+                        const double lF[8][3], double lX[8], double lY[8], double lZ[8],
+                        double lVX[8], double lVY[8], double lVZ[8]) {
+  HgArraySrc src{{VX, VY, VZ}, lF};
+  HgArraySink sink{{lVX, lVY, lVZ}};
+  if (q2)
+    return hourglass_rev_s<true>(X, Y, Z, src, sink, determ, ss, mass, hourg, cbrt_determ, lX, lY, lZ);
+  return hourglass_rev_s<false>(X, Y, Z, src, sink, determ, ss, mass, hourg, cbrt_determ, lX, lY, lZ);
 }
 
 // ---------------------------------------------------------------------------
@@ -340,9 +457,11 @@ LHD double monoq_grad_rev(const double X[8], const double Y[8], const double Z[8
   const double norm = 1.0 / (vol + ptiny);
   double D[3][3];  // D[d] = difference vector of direction d (i, j, k)
   double DV[3][3];
+  LUNROLL
   for (int d = 0; d < 3; ++d) {
     D[d][0] = D[d][1] = D[d][2] = 0.0;
     DV[d][0] = DV[d][1] = DV[d][2] = 0.0;
+    LUNROLL
     for (int n = 0; n < 8; ++n) {
       const double w = dir_coef(d, n);
       D[d][0] += w * X[n]; D[d][1] += w * Y[n]; D[d][2] += w * Z[n];
@@ -353,6 +472,7 @@ LHD double monoq_grad_rev(const double X[8], const double Y[8], const double Z[8
   const int U[3] = {1, 2, 0}, W[3] = {2, 0, 1}, DVd[3] = {0, 1, 2};
   double lD[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, lDV[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   double lvol = 0.0, lnorm = 0.0;
+  LUNROLL
   for (int o = 0; o < 3; ++o) {
     double a[3];
     cross(D[U[o]], D[W[o]], a);
@@ -363,6 +483,7 @@ LHD double monoq_grad_rev(const double X[8], const double Y[8], const double Z[8
     const double adv = a[0] * dv[0] + a[1] * dv[1] + a[2] * dv[2];
     double la[3];
     lvol += ldelx / r;
+    LUNROLL
     for (int c = 0; c < 3; ++c) {
       la[c] = -ldelx * vol / (r2 * r) * a[c] + ldelv * norm * dv[c];
       lDV[DVd[o]][c] += ldelv * norm * a[c];
@@ -371,12 +492,15 @@ LHD double monoq_grad_rev(const double X[8], const double Y[8], const double Z[8
     double lu[3], lw[3];
     cross(D[W[o]], la, lu);  // a = u x w
     cross(la, D[U[o]], lw);
+    LUNROLL
     for (int c = 0; c < 3; ++c) {
       lD[U[o]][c] += lu[c];
       lD[W[o]][c] += lw[c];
     }
   }
+  LUNROLL
   for (int d = 0; d < 3; ++d)
+    LUNROLL
     for (int n = 0; n < 8; ++n) {
       const double w = dir_coef(d, n);
       lX[n] += w * lD[d][0]; lY[n] += w * lD[d][1]; lZ[n] += w * lD[d][2];
@@ -430,6 +554,7 @@ LHD QOut monoq_region_rev(const QIn &in, double lqlin, double lqquad, double ldv
   if (in.vdov > 0.0) return o;
   double phi[3], g[3][3], dvx[3];
   bool act[3];
+  LUNROLL
   for (int d = 0; d < 3; ++d) {
     phi[d] = phi_eval(in.dv[d], in.dm[d], in.dp[d], in.mult, in.maxslope, g[d]);
     dvx[d] = in.dv[d] * in.dx[d];
@@ -438,6 +563,7 @@ LHD QOut monoq_region_rev(const QIn &in, double lqlin, double lqquad, double ldv
   }
   const double rho = in.elemMass / (in.volo * in.vnew);
   double S1 = 0.0, S2 = 0.0;
+  LUNROLL
   for (int d = 0; d < 3; ++d) {
     S1 += dvx[d] * (1.0 - phi[d]);
     S2 += dvx[d] * dvx[d] * (1.0 - phi[d] * phi[d]);
@@ -447,6 +573,7 @@ LHD QOut monoq_region_rev(const QIn &in, double lqlin, double lqquad, double ldv
   const double lrho = lqlin * (-in.qlc * S1) + lqquad * (in.qqc * S2);
   *lqlc += lqlin * (-rho * S1);
   *lqqc += lqquad * (rho * S2);
+  LUNROLL
   for (int d = 0; d < 3; ++d) {
     const double ldvx = lqlin * (-in.qlc * rho) * (1.0 - phi[d]) +
                         lqquad * (in.qqc * rho) * 2.0 * dvx[d] * (1.0 - phi[d] * phi[d]);

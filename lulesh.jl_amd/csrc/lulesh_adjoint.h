@@ -7,8 +7,10 @@ namespace lulesh {
 
 struct AdjArgs {
   DevScalars *sc;
-  DevFields f;       // instance 0 (offsets are zero for a one-instance context)
+  DevFields f;       // fields of cycle c: e p q v ss of S_c, its temporaries (vnew delv
+                     // vdov delx_* delv_*), and volo elemMass nodalMass elemBC
   NodeSet cur, old;  // node set holding x', xd' and the one holding S_c
+  double dt;         // the cycle's time step
   int s;
   int64_t E, N;
   bool q1, q2, q3;
@@ -30,6 +32,11 @@ hipError_t launch_set_dt(hipStream_t st, DevScalars *sc, const double *dt, int64
 hipError_t launch_record_dt(hipStream_t st, DevScalars *sc, double *dt, int64_t base);
 hipError_t launch_state_copy(hipStream_t st, DevScalars *sc, const DevFields &f, int64_t E,
                              int64_t N, double *buf, int to_buf);
+// after a cycle: the current node set (x' .. zd', 6N) and the temporaries
+// vnew delv vdov delx_xi delx_eta delx_zeta delv_xi delv_eta delv_zeta (9E)
+// into buf, in that order
+hipError_t launch_store_temps(hipStream_t st, DevScalars *sc, const DevFields &f, int64_t E,
+                              int64_t N, double *buf);
 hipError_t launch_set_cycle(hipStream_t st, DevScalars *sc, long long cycle);
 
 }  // namespace lulesh

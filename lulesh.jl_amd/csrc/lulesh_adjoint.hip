@@ -246,7 +246,7 @@ __global__ __launch_bounds__(BLOCK) void a_node1(AdjArgs A) {
   if (n >= A.N) return;
   double g[6];
   gather_corners(A, n, g);
-  const double dt = A.sc->deltatime;
+  const double dt = A.dt;
   const int64_t s1 = A.s + 1;
   const int i = (int)(n % s1), j = (int)((n / s1) % s1), k = (int)(n / (s1 * s1));
   const double mass = A.f.nodalMass[n];
@@ -270,6 +270,25 @@ __global__ __launch_bounds__(BLOCK) void a_node1(AdjArgs A) {
 }
 
 // ---- reverse of the corner forces (stress + FB hourglass) -------------------
+// velocities of S_c and force adjoints read from memory node by node, so
+// they do not occupy registers across the whole element
+struct NodeSrc {
+  const double *vx, *vy, *vz, *lF;
+  int64_t N;
+  const int64_t *nid;
+  LDEV double v(int c, int n) const {
+    return (c == 0 ? vx : (c == 1 ? vy : vz))[nid[n]];
+  }
+  LDEV double lf(int n, int c) const { return lF[(int64_t)c * N + nid[n]]; }
+};
+// per-corner velocity adjoints straight to the corner buffer
+struct CornerSink {
+  double *cr;
+  int64_t E, e;
+  LDEV void lv(int c, int n, double x) { cr[(int64_t)((3 + c) * 8 + n) * E + e] = x; }
+};
+
+template <bool Q1, bool Q2>
 __global__ __launch_bounds__(BLOCK) void a_force(AdjArgs A) {
   const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (e >= A.E) return;
@@ -277,36 +296,36 @@ __global__ __launch_bounds__(BLOCK) void a_force(AdjArgs A) {
   const NodeSet S = A.old;
   int64_t nid[8];
   elem_nodes_a(e, A.s, nid);
-  double X[8], Y[8], Z[8], VX[8], VY[8], VZ[8], lF[8][3];
+  double X[8], Y[8], Z[8];
   for (int n = 0; n < 8; ++n) {
     X[n] = S.x[nid[n]]; Y[n] = S.y[nid[n]]; Z[n] = S.z[nid[n]];
-    VX[n] = S.xd[nid[n]]; VY[n] = S.yd[nid[n]]; VZ[n] = S.zd[nid[n]];
-    for (int c = 0; c < 3; ++c) lF[n][c] = A.lF[(int64_t)c * A.N + nid[n]];
   }
-  double lX[8] = {0}, lY[8] = {0}, lZ[8] = {0}, lVX[8] = {0}, lVY[8] = {0}, lVZ[8] = {0};
-  const double ms = f.p[e] + f.q[e];
-  const double lms = lulesh_adj::stress_rev(X, Y, Z, A.q1, ms, lF, lX, lY, lZ);
-  A.lp[e] += lms;
-  A.lq[e] += lms;
+  const NodeSrc src{S.xd, S.yd, S.zd, A.lF, A.N, nid};
+  CornerSink sink{A.corner, A.E, e};
+  double lX[8] = {0}, lY[8] = {0}, lZ[8] = {0};
   const double hourg = A.sc->hgcoef;
   if (hourg > 0.0) {
     const double volo = f.volo[e];
     const double determ = volo * f.v[e];
-    const lulesh_adj::HgRev r = lulesh_adj::hourglass_rev(
-        X, Y, Z, VX, VY, VZ, determ, f.ss[e], f.elemMass[e], hourg, cbrt_cr(determ), A.q2, lF, lX,
-        lY, lZ, lVX, lVY, lVZ);
+    const double ss = f.ss[e], mass = f.elemMass[e], c13 = cbrt_cr(determ);
+    const lulesh_adj::HgRev r = lulesh_adj::hourglass_rev_s<Q2>(X, Y, Z, src, sink, determ, ss,
+                                                                mass, hourg, c13, lX, lY, lZ);
     A.lv[e] += r.ldeterm * volo;
     A.lss[e] += r.lss;
     A.pacc[e] += r.lhgcoef;
+  } else {
+    for (int c = 0; c < 3; ++c)
+      for (int n = 0; n < 8; ++n) sink.lv(c, n, 0.0);
   }
+  const double ms = f.p[e] + f.q[e];
+  const double lms = lulesh_adj::stress_rev_s(X, Y, Z, Q1, ms, src, lX, lY, lZ);
+  A.lp[e] += lms;
+  A.lq[e] += lms;
   double *cr = A.corner;
   for (int n = 0; n < 8; ++n) {
     cr[(int64_t)(0 * 8 + n) * A.E + e] = lX[n];
     cr[(int64_t)(1 * 8 + n) * A.E + e] = lY[n];
     cr[(int64_t)(2 * 8 + n) * A.E + e] = lZ[n];
-    cr[(int64_t)(3 * 8 + n) * A.E + e] = lVX[n];
-    cr[(int64_t)(4 * 8 + n) * A.E + e] = lVY[n];
-    cr[(int64_t)(5 * 8 + n) * A.E + e] = lVZ[n];
   }
 }
 
@@ -375,6 +394,21 @@ __global__ __launch_bounds__(BLOCK) void a_state_copy(DevScalars *sc, DevFields 
     }
 }
 
+__global__ __launch_bounds__(BLOCK) void a_store_temps(DevScalars *sc, DevFields f, int64_t E,
+                                                       int64_t N, double *buf) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int par = sc->parity;
+  const double *nd[6] = {par ? f.ns[1].x : f.ns[0].x, par ? f.ns[1].y : f.ns[0].y,
+                         par ? f.ns[1].z : f.ns[0].z, par ? f.ns[1].xd : f.ns[0].xd,
+                         par ? f.ns[1].yd : f.ns[0].yd, par ? f.ns[1].zd : f.ns[0].zd};
+  const double *el[9] = {f.vnew, f.delv, f.vdov, f.delx_xi, f.delx_eta, f.delx_zeta,
+                         f.delv_xi, f.delv_eta, f.delv_zeta};
+  if (i < N)
+    for (int k = 0; k < 6; ++k) buf[k * N + i] = nd[k][i];
+  if (i < E)
+    for (int k = 0; k < 9; ++k) buf[6 * N + k * E + i] = el[k][i];
+}
+
 __global__ void a_set_cycle(DevScalars *sc, long long cycle) {
   if (threadIdx.x == 0 && blockIdx.x == 0) sc->cycle = cycle;
 }
@@ -393,7 +427,14 @@ hipError_t launch_adjoint(int which, hipStream_t st, const AdjArgs &a) {
       hipLaunchKernelGGL(a_node1, dim3(nblk_a(a.N)), dim3(BLOCK), 0, st, a);
       break;
     case AK_FORCE:
-      hipLaunchKernelGGL(a_force, dim3(nblk_a(a.E)), dim3(BLOCK), 0, st, a);
+      if (a.q1 && a.q2)
+        hipLaunchKernelGGL((a_force<true, true>), dim3(nblk_a(a.E)), dim3(BLOCK), 0, st, a);
+      else if (a.q1)
+        hipLaunchKernelGGL((a_force<true, false>), dim3(nblk_a(a.E)), dim3(BLOCK), 0, st, a);
+      else if (a.q2)
+        hipLaunchKernelGGL((a_force<false, true>), dim3(nblk_a(a.E)), dim3(BLOCK), 0, st, a);
+      else
+        hipLaunchKernelGGL((a_force<false, false>), dim3(nblk_a(a.E)), dim3(BLOCK), 0, st, a);
       break;
     case AK_NODE2:
       hipLaunchKernelGGL(a_node2, dim3(nblk_a(a.N)), dim3(BLOCK), 0, st, a);
@@ -411,6 +452,13 @@ hipError_t launch_state_copy(hipStream_t st, DevScalars *sc, const DevFields &f,
                              int64_t N, double *buf, int to_buf) {
   hipLaunchKernelGGL(a_state_copy, dim3(nblk_a(N > E ? N : E)), dim3(BLOCK), 0, st, sc, f, E, N,
                      buf, to_buf);
+  return hipGetLastError();
+}
+
+hipError_t launch_store_temps(hipStream_t st, DevScalars *sc, const DevFields &f, int64_t E,
+                              int64_t N, double *buf) {
+  hipLaunchKernelGGL(a_store_temps, dim3(nblk_a(N > E ? N : E)), dim3(BLOCK), 0, st, sc, f, E, N,
+                     buf);
   return hipGetLastError();
 }
 

@@ -499,11 +499,26 @@ int adj_alloc(lulesh_ctx *c, double **p, int64_t n) {
   return 0;
 }
 
+// one cycle of a segment: its start state S_k (6N + 5E, the checkpoint
+// layout), the node set after it (6N) and its temporaries (9E)
+int64_t cycle_slot(const lulesh_ctx *c) {
+  const int64_t E = c->inst[0].E, N = c->inst[0].N;
+  return 12 * N + 14 * E;
+}
+
+// dt schedule of `ncycles`; with `every` > 0 also the adjoint fields, the
+// checkpoints and one segment of `every` cycle slots
 int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
   AdjBuffers &B = c->adj;
   const int64_t E = c->inst[0].E, N = c->inst[0].N;
   const int64_t state = 6 * N + 5 * E;
   int rc = 0;
+  if (ncycles > B.dt_cap) {
+    rc = adj_alloc(c, &B.dt, ncycles);
+    if (rc) return rc;
+    B.dt_cap = ncycles;
+  }
+  if (every <= 0) return 0;
   if (!B.ready) {
     AdjArgs &a = B.a;
     a.sc = c->b.sc;
@@ -533,12 +548,12 @@ int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
       B.mem.push_back(B.final_sc);
     }
     if (rc) return rc;
+    // the shadow of lulesh_vjp_cycle starts at zero
+    double *zero[] = {a.lx, a.ly, a.lz, a.lxd, a.lyd, a.lzd};
+    for (double *p : zero) HIP_TRY(c, hipMemset(p, 0, sizeof(double) * N));
+    double *zeroe[] = {a.le, a.lp, a.lq, a.lv, a.lss};
+    for (double *p : zeroe) HIP_TRY(c, hipMemset(p, 0, sizeof(double) * E));
     B.ready = true;
-  }
-  if (ncycles > B.dt_cap) {
-    rc = adj_alloc(c, &B.dt, ncycles);
-    if (rc) return rc;
-    B.dt_cap = ncycles;
   }
   const int64_t nck = (ncycles + every - 1) / every;
   if (nck > B.ckpt_cap) {
@@ -547,7 +562,7 @@ int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
     B.ckpt_cap = nck;
   }
   if (every > B.seg_cap) {
-    rc = adj_alloc(c, &B.seg, every * state);
+    rc = adj_alloc(c, &B.seg, every * cycle_slot(c));
     if (rc) return rc;
     B.seg_cap = every;
   }
@@ -574,11 +589,38 @@ int replay_cycle(lulesh_ctx *c, const double *dt, int64_t base, bool constraints
   return 0;
 }
 
-AdjArgs adj_args(lulesh_ctx *c, int parity) {
+// adjoint kernel arguments reading cycle k from its stored slot
+AdjArgs stored_args(lulesh_ctx *c, const double *slot, double dt) {
+  const int64_t E = c->inst[0].E, N = c->inst[0].N;
   AdjArgs a = c->adj.a;
-  a.cur = c->b.f.ns[parity & 1];
-  a.old = c->b.f.ns[(parity & 1) ^ 1];
+  double *p = const_cast<double *>(slot);
+  a.old = NodeSet{p, p + N, p + 2 * N, p + 3 * N, p + 4 * N, p + 5 * N};
+  double *el = p + 6 * N;
+  a.f.e = el; a.f.p = el + E; a.f.q = el + 2 * E; a.f.v = el + 3 * E; a.f.ss = el + 4 * E;
+  double *nw = el + 5 * E;
+  a.cur = NodeSet{nw, nw + N, nw + 2 * N, nw + 3 * N, nw + 4 * N, nw + 5 * N};
+  double *t = nw + 6 * N;
+  a.f.vnew = t; a.f.delv = t + E; a.f.vdov = t + 2 * E;
+  a.f.delx_xi = t + 3 * E; a.f.delx_eta = t + 4 * E; a.f.delx_zeta = t + 5 * E;
+  a.f.delv_xi = t + 6 * E; a.f.delv_eta = t + 7 * E; a.f.delv_zeta = t + 8 * E;
+  a.dt = dt;
   return a;
+}
+
+// the five adjoint kernels of one cycle
+int adjoint_cycle(lulesh_ctx *c, const AdjArgs &aa) {
+  const int ks[] = {AK_QEOS, AK_KIN, AK_NODE1, AK_FORCE, AK_NODE2};
+  for (int kk : ks) {
+    hipError_t e = launch_adjoint(kk, c->stream, aa);
+    if (e != hipSuccess) return hip_fail(c, e, "adjoint kernel");
+  }
+  return 0;
+}
+
+int store_temps(lulesh_ctx *c, double *slot) {
+  const int64_t E = c->inst[0].E, N = c->inst[0].N;
+  hipError_t e = launch_store_temps(c->stream, c->b.sc, c->b.f, E, N, slot + 6 * N + 5 * E);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "store temporaries");
 }
 
 int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, double grad_params[3],
@@ -623,18 +665,16 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
   rc = state_copy(c, B.final_state, 1);
   if (rc) return rc;
   HIP_TRY(c, hipMemcpyAsync(B.final_sc, c->b.sc, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream));
-  // 2. reverse sweep, segment by segment
+  // 2. reverse sweep, segment by segment: restore the segment's checkpoint,
+  // replay it storing each cycle's start state and temporaries, then run
+  // the adjoint kernels of its cycles backwards against the stored slots
+  std::vector<double> hdt(ncycles);
+  HIP_TRY(c, hipMemcpy(hdt.data(), B.dt, sizeof(double) * ncycles, hipMemcpyDeviceToHost));
   {
-    hipError_t e = launch_adjoint(AK_SEED, c->stream, adj_args(c, 0));
+    hipError_t e = launch_adjoint(AK_SEED, c->stream, B.a);
     if (e != hipSuccess) return hip_fail(c, e, "adjoint seed");
   }
-  // every k_flip toggles the node-set parity; the host follows it so the
-  // reverse sweep needs no synchronisation
-  int parity = 0;
-  rc = pull_scalars(c);
-  if (!rc) rc = finish(c, false);
-  if (rc) return rc;
-  parity = c->hsc[0].parity;
+  const int64_t slot = cycle_slot(c);
   const int64_t nseg = (ncycles + every - 1) / every;
   for (int64_t sg = nseg - 1; sg >= 0 && !rc; --sg) {
     const int64_t k0 = sg * every, k1 = (k0 + every < ncycles) ? k0 + every : ncycles;
@@ -644,33 +684,13 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
       if (e != hipSuccess) rc = hip_fail(c, e, "set cycle");
     }
     for (int64_t k = k0; k < k1 && !rc; ++k) {
-      rc = state_copy(c, B.seg + (k - k0) * state, 1);
+      double *sl = B.seg + (k - k0) * slot;
+      rc = state_copy(c, sl, 1);
       if (!rc) rc = replay_cycle(c, B.dt, cycle0, false);
-      parity ^= 1;
+      if (!rc) rc = store_temps(c, sl);
     }
-    for (int64_t k = k1 - 1; k >= k0 && !rc; --k) {
-      // S_k back in place, then the forward nodal + kinematics of cycle k
-      rc = state_copy(c, B.seg + (k - k0) * state, 0);
-      if (!rc) {
-        hipError_t e = launch_set_cycle(c->stream, c->b.sc, cycle0 + k);
-        if (e == hipSuccess) e = launch_set_dt(c->stream, c->b.sc, B.dt, cycle0);
-        if (e != hipSuccess) rc = hip_fail(c, e, "set dt");
-      }
-      if (!rc) rc = launch(c, KID_NODAL, a);
-      if (!rc) rc = launch(c, KID_FLIP, a);
-      if (!rc) rc = launch(c, KID_KIN, a);
-      if (rc) break;
-      parity ^= 1;
-      const AdjArgs aa = adj_args(c, parity);
-      const int ks[] = {AK_QEOS, AK_KIN, AK_NODE1, AK_FORCE, AK_NODE2};
-      for (int kk : ks) {
-        hipError_t e = launch_adjoint(kk, c->stream, aa);
-        if (e != hipSuccess) {
-          rc = hip_fail(c, e, "adjoint kernel");
-          break;
-        }
-      }
-    }
+    for (int64_t k = k1 - 1; k >= k0 && !rc; --k)
+      rc = adjoint_cycle(c, stored_args(c, B.seg + (k - k0) * slot, hdt[k]));
   }
   if (rc) return rc;
   // 3. results, then the state the forward pass ended in
@@ -692,9 +712,97 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
   return rc;
 }
 
+// one lagrangeLeapFrog with the shadow mapped from the adjoint of the state
+// after it to that of the state before it (Enzyme's Duplicated semantics)
+int vjp_cycle(lulesh_ctx *c, double grad_params[3]) {
+  if (c->n != 1) return fail(c, LULESH_ERR_UNSUPPORTED, "the adjoint needs a one-domain context");
+  HIP_TRY(c, hipSetDevice(c->device));
+  int rc = adj_setup(c, 1, 1);
+  if (rc) return rc;
+  AdjBuffers &B = c->adj;
+  const int64_t E = c->inst[0].E;
+  const Batch a = make_args(c, 0, 0);
+  // the forward cycle exactly as lulesh_lagrange_leapfrog runs it, with its
+  // start state and temporaries kept in a cycle slot
+  rc = state_copy(c, B.seg, 1);
+  if (!rc) rc = clear_err(c);
+  const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
+  for (int id : ids)
+    if (!rc) rc = launch(c, id, a);
+  if (!rc) rc = store_temps(c, B.seg);
+  if (!rc) rc = finish(c, true);
+  if (rc) return rc;
+  c->partials_valid = false;
+  HIP_TRY(c, hipMemsetAsync(B.a.pacc, 0, sizeof(double) * 3 * E, c->stream));
+  rc = adjoint_cycle(c, stored_args(c, B.seg, c->hsc[0].deltatime));
+  if (rc) return rc;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (grad_params) {
+    std::vector<double> pa(3 * E);
+    HIP_TRY(c, hipMemcpy(pa.data(), B.a.pacc, sizeof(double) * 3 * E, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) {
+      double sum = 0.0;
+      for (int64_t i = 0; i < E; ++i) sum += pa[k * E + i];
+      grad_params[k] += sum;
+    }
+  }
+  return 0;
+}
+
+// shadow buffer of one state field, or nullptr
+double *shadow_field(lulesh_ctx *c, const char *name, int64_t *n) {
+  if (!name) return nullptr;
+  AdjArgs &a = c->adj.a;
+  struct Entry {
+    const char *name;
+    double *p;
+    bool node;
+  };
+  const Entry tab[] = {{"x", a.lx, true},   {"y", a.ly, true},   {"z", a.lz, true},
+                       {"xd", a.lxd, true}, {"yd", a.lyd, true}, {"zd", a.lzd, true},
+                       {"e", a.le, false},  {"p", a.lp, false},  {"q", a.lq, false},
+                       {"v", a.lv, false},  {"ss", a.lss, false}};
+  for (const Entry &t : tab)
+    if (std::strcmp(t.name, name) == 0) {
+      *n = t.node ? c->inst[0].N : c->inst[0].E;
+      return t.p;
+    }
+  return nullptr;
+}
+
+int shadow_move(lulesh_ctx *c, const char *field, double *host, int64_t n, bool up) {
+  if (c->n != 1) return fail(c, LULESH_ERR_UNSUPPORTED, "the adjoint needs a one-domain context");
+  if (!host) return fail(c, LULESH_ERR_INVALID_ARG, "null buffer");
+  HIP_TRY(c, hipSetDevice(c->device));
+  int rc = adj_setup(c, 1, 1);
+  if (rc) return rc;
+  int64_t len = 0;
+  double *p = shadow_field(c, field, &len);
+  if (!p) return fail(c, LULESH_ERR_INVALID_ARG, std::string("no shadow for field ") + (field ? field : "(null)"));
+  if (n != len) return fail(c, LULESH_ERR_INVALID_ARG, "shadow field length mismatch");
+  if (up) HIP_TRY(c, hipMemcpy(p, host, sizeof(double) * n, hipMemcpyHostToDevice));
+  else HIP_TRY(c, hipMemcpy(host, p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int lulesh_vjp_cycle(lulesh_ctx *c, double *grad_params) {
+  if (!c) return LULESH_ERR_INVALID_ARG;
+  return vjp_cycle(c, grad_params);
+}
+
+int lulesh_shadow_upload(lulesh_ctx *c, const char *field, const double *src, int64_t n) {
+  if (!c) return LULESH_ERR_INVALID_ARG;
+  return shadow_move(c, field, const_cast<double *>(src), n, true);
+}
+
+int lulesh_shadow_download(lulesh_ctx *c, const char *field, double *dst, int64_t n) {
+  if (!c) return LULESH_ERR_INVALID_ARG;
+  return shadow_move(c, field, dst, n, false);
+}
 
 int lulesh_grad_run(lulesh_ctx *c, int64_t ncycles, int64_t checkpoint_every, double *grad_e,
                     double *grad_params, double *loss, double *dt_schedule) {
@@ -706,7 +814,7 @@ int lulesh_replay(lulesh_ctx *c, const double *dt, int64_t ncycles) {
   if (!c || !dt || ncycles < 0) return LULESH_ERR_INVALID_ARG;
   if (c->n != 1) return fail(c, LULESH_ERR_UNSUPPORTED, "replay needs a one-domain context");
   HIP_TRY(c, hipSetDevice(c->device));
-  int rc = adj_setup(c, ncycles > 0 ? ncycles : 1, 1);
+  int rc = adj_setup(c, ncycles > 0 ? ncycles : 1, 0);
   if (rc) return rc;
   HIP_TRY(c, hipMemcpy(c->adj.dt, dt, sizeof(double) * ncycles, hipMemcpyHostToDevice));
   rc = pull_scalars(c);
@@ -957,6 +1065,12 @@ int lulesh_run(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
 int lulesh_batch_run(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
   if (!c) return LULESH_ERR_INVALID_ARG;
   return run_cycles(c, num_iters, cycles_run);
+}
+
+int lulesh_batch_error(const lulesh_ctx *c, int32_t inst) {
+  if (!c || inst < 0 || inst >= c->n) return LULESH_ERR_INVALID_ARG;
+  const int err = c->hsc[inst].err;
+  return (err >= 1 && err <= 9) ? err : 0;
 }
 
 int lulesh_batch_verify(lulesh_ctx *c, int32_t inst, double out[4]) {

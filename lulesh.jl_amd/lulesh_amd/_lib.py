@@ -7,7 +7,10 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "liblulesh_hip.so"))
+# LULESH_HIP_LIBRARY selects another build of the library (a variant built
+# with different kernel options, for instance)
+LIB_PATH = os.environ.get("LULESH_HIP_LIBRARY") or os.path.normpath(
+    os.path.join(_HERE, "..", "lib", "liblulesh_hip.so"))
 
 c_int, c_int32, c_int64, c_double = ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
 c_char_p, c_void_p = ctypes.c_char_p, ctypes.c_void_p
@@ -63,6 +66,10 @@ SIGNATURES = {
     "lulesh_grad_run": (c_int, [c_void_p, c_int64, c_int64, P_double, P_double, P_double,
                                 P_double]),
     "lulesh_replay": (c_int, [c_void_p, P_double, c_int64]),
+    "lulesh_vjp_cycle": (c_int, [c_void_p, P_double]),
+    "lulesh_batch_error": (c_int, [c_void_p, c_int32]),
+    "lulesh_shadow_upload": (c_int, [c_void_p, c_char_p, P_double, c_int64]),
+    "lulesh_shadow_download": (c_int, [c_void_p, c_char_p, P_double, c_int64]),
 }
 
 _lib = None

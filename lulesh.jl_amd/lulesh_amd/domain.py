@@ -273,6 +273,46 @@ def replay(domain, dt):
            domain._lib.lulesh_replay(domain._ctx, a.ctypes.data_as(_lib.P_double), a.size))
 
 
+SHADOW_FIELDS = ("x", "y", "z", "xd", "yd", "zd", "e", "p", "q", "v", "ss")
+
+
+def set_shadow(domain, field, values):
+    """Load one field of the context's shadow (adjoint of that state field)."""
+    a = np.ascontiguousarray(values, dtype=np.float64)
+    _check(domain._lib, domain._ctx, domain._lib.lulesh_shadow_upload(
+        domain._ctx, field.encode(), a.ctypes.data_as(_lib.P_double), a.size))
+
+
+def get_shadow(domain, field):
+    n = domain.numNode if field in ("x", "y", "z", "xd", "yd", "zd") else domain.numElem
+    out = np.empty(n, dtype=np.float64)
+    _check(domain._lib, domain._ctx, domain._lib.lulesh_shadow_download(
+        domain._ctx, field.encode(), out.ctypes.data_as(_lib.P_double), n))
+    return out
+
+
+def vjp_cycle(domain, shadow=None, grad_params=None):
+    """One lagrangeLeapFrog under reverse mode, the counterpart of
+    `Enzyme.autodiff(lagrangeLeapFrog, Duplicated(domain, shadowDomain))`
+    (examples/benchmark.jl:86): the domain advances one cycle and the shadow
+    maps from the adjoint of the state after the cycle to the adjoint of the
+    state before it.  `shadow` (dict field -> array over SHADOW_FIELDS, or
+    None to use the device-resident shadow as it stands) is loaded first and
+    the returned dict holds the new shadow; `grad_params` (3 floats: hgcoef,
+    qlc_monoq, qqc_monoq) accumulates."""
+    if shadow is not None:
+        for f in SHADOW_FIELDS:
+            if f in shadow:
+                set_shadow(domain, f, shadow[f])
+    gp = np.zeros(3) if grad_params is None else np.asarray(grad_params, dtype=np.float64)
+    buf = np.ascontiguousarray(gp)
+    _check(domain._lib, domain._ctx,
+           domain._lib.lulesh_vjp_cycle(domain._ctx, buf.ctypes.data_as(_lib.P_double)))
+    if grad_params is not None:
+        grad_params[:] = buf
+    return {f: get_shadow(domain, f) for f in SHADOW_FIELDS}, buf
+
+
 def total_internal_energy(domain):
     e, m = domain.e, domain.elemMass
     total = 0.0
@@ -329,11 +369,21 @@ class Batch:
         except Exception:
             pass
 
-    def run(self, num_iters):
-        """Driver loop for every instance; returns cycles run per instance."""
+    def run(self, num_iters, check=True):
+        """Driver loop for every instance; returns cycles run per instance.
+        An instance that raises a reference error stops while the others run
+        on; with check=True the lowest such error is raised afterwards,
+        otherwise `errors()` tells which instances stopped that way."""
         out = (ctypes.c_int64 * self.n)()
-        _check(self._lib, self._ctx, self._lib.lulesh_batch_run(self._ctx, int(num_iters), out))
+        rc = self._lib.lulesh_batch_run(self._ctx, int(num_iters), out)
+        if check or not (1 <= rc <= 9):
+            _check(self._lib, self._ctx, rc)
         return np.array(out[:], dtype=np.int64)
+
+    def errors(self):
+        """Per instance: the reference error code that stopped it in the last
+        run (0 if none)."""
+        return [int(self._lib.lulesh_batch_error(self._ctx, i)) for i in range(self.n)]
 
     def verify(self, i):
         return verify(self.views[i])
