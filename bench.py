@@ -121,11 +121,14 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=192)
-    ap.add_argument("--cpu-cycles", type=int, default=3)
+    ap.add_argument("--cpu-cycles", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--only", choices=("adjoint", "batched"), default=None,
+                    help="run only the adjoint-ratio or the batched-speedup measurement and print "
+                         "its JSON object (for profiling them on their own)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the "
                          "reference point for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -144,6 +147,9 @@ def main():
         print("bench.py: this build runs one single-GPU domain; multi-GPU (--gpus > 1) is not "
               "part of it.", file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        print(json.dumps(bench_adjoint() if args.only == "adjoint" else bench_batched()))
+        return
 
     from lulesh_amd import Domain, LuleshProblem, run
 

@@ -63,7 +63,8 @@ typedef struct lulesh_ctx lulesh_ctx;
  * src/LULESH.jl:31-49): builds the Sedov problem with nx elements per edge
  * on the device.  nr/balance/cost are accepted as in LuleshProblem but, as in
  * the reference, never reach the arithmetic (the compute loops walk the
- * identity matElemlist, src/domain.jl:657-661).  `device` is a HIP ordinal. */
+ * identity matElemlist, src/domain.jl:657-661).  `device` is a HIP ordinal.
+ * nx must lie in 1..800 (LULESH_ERR_INVALID_ARG otherwise). */
 int lulesh_create(int32_t nx, int32_t nr, int32_t balance, int32_t cost, int32_t quirks,
                   int32_t device, lulesh_ctx **out);
 int lulesh_destroy(lulesh_ctx *ctx);

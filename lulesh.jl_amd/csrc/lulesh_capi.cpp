@@ -311,7 +311,10 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
   *out = nullptr;
   if (n < 1 || !nx || quirks < 0 || quirks > LULESH_QUIRKS_REFERENCE) return LULESH_ERR_INVALID_ARG;
   for (int i = 0; i < n; ++i) {
-    if (nx[i] < 1 || nx[i] > 1200) return LULESH_ERR_INVALID_ARG;
+    // kernels address an instance's node and element arrays with 32-bit byte
+    // offsets: (nx+1)^3 * 8 < 2^32 holds up to nx = 811 (s = 800 already
+    // needs ~240 GB of HBM)
+    if (nx[i] < 1 || nx[i] > 800) return LULESH_ERR_INVALID_ARG;
     if (einit_scale && !(einit_scale[i] > 0.0)) return LULESH_ERR_INVALID_ARG;
   }
   lulesh_ctx *c = new lulesh_ctx();

@@ -46,6 +46,40 @@ LDEV void gather8(const T *__restrict__ src, const int64_t (&nid)[8], T (&out)[8
   for (int n = 0; n < 8; ++n) out[n] = src[nid[n]];
 }
 
+// An element's 8 nodes as 4 adjacent pairs (n, n+1) -- nodes (0,1) (3,2)
+// (4,5) (7,6) of the hex order of buildMesh! -- each pair one 16-byte load
+// at a 32-bit byte offset from a uniform base (node arrays of a context stay
+// below 4 GiB).  o[] = byte offsets of nodes 0, 3, 4, 7.
+LDEV void hex_pairs(uint32_t n0, uint32_t s1, uint32_t (&o)[4]) {
+  const uint32_t P = s1 * s1;
+  o[0] = n0 * 8u;
+  o[1] = (n0 + s1) * 8u;
+  o[2] = (n0 + P) * 8u;
+  o[3] = (n0 + P + s1) * 8u;
+}
+
+LDEV double2 ld_pair(const double *base, uint32_t boff) {
+  const double *p = (const double *)__builtin_assume_aligned((const char *)base + boff, 8);
+  double2 v;
+  __builtin_memcpy(&v, p, 16);
+  return v;
+}
+
+LDEV void gather8p(const double *__restrict__ base, const uint32_t (&o)[4], double (&out)[8]) {
+  const double2 a = ld_pair(base, o[0]), b = ld_pair(base, o[1]), c = ld_pair(base, o[2]),
+                d = ld_pair(base, o[3]);
+  out[0] = a.x; out[1] = a.y;
+  out[3] = b.x; out[2] = b.y;
+  out[4] = c.x; out[5] = c.y;
+  out[7] = d.x; out[6] = d.y;
+}
+
+// element of a field at a 32-bit byte offset from a uniform base
+template <typename T>
+LDEV T &at(T *base, uint32_t boff) {
+  return *(T *)((char *)base + boff);
+}
+
 LDEV void raise(DevScalars *sc, int code) { atomicMin(&sc->err, code); }
 
 // instance i's view of the flat field arrays
@@ -343,25 +377,27 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
     double hgf[8][3];
     if (plane && elem_col) {
       const int64_t e = ((int64_t)ek * s + ej) * s + ei;
-      int64_t nid[8];
-      elem_nodes(e, s, nid);
+      const uint32_t eo = (uint32_t)e * 8u;
+      uint32_t nid[4];
+      hex_pairs(((uint32_t)ek * (uint32_t)s1 + (uint32_t)ej) * (uint32_t)s1 + (uint32_t)ei,
+                (uint32_t)s1, nid);
       double X[8], Y[8], Z[8];
-      gather8(in.x, nid, X); gather8(in.y, nid, Y); gather8(in.z, nid, Z);
+      gather8p(in.x, nid, X); gather8p(in.y, nid, Y); gather8p(in.z, nid, Z);
       // stress corners, src/domain.jl:1097-1108 (sig from 963-969)
       const double detJ = jacobian_volume(X, Y, Z);
       if (detJ <= 0.0) raise(sc, ERR_EARLY_VOLUME);
       {
         double B[8][3];
         node_normals(X, Y, Z, quirks, B);
-        const double ms = -((-f.p[e]) - f.q[e]);
+        const double ms = -((-at(f.p, eo)) - at(f.q, eo));
 #pragma unroll
         for (int n = 0; n < 8; ++n)
 #pragma unroll
           for (int c = 0; c < 3; ++c) lds[(n * 3 + c) * BLOCK + t] = ms * B[n][c];
       }
       // hourglass corners, src/domain.jl:1707-1729 and 1505-1639
-      const double v = f.v[e];
-      const double determ = f.volo[e] * v;
+      const double v = at(f.v, eo);
+      const double determ = at(f.volo, eo) * v;
       if (v <= 0.0) raise(sc, ERR_HG_VOLUME);
       if (hg_on) {
         // Mode-major form of the same arithmetic: for each hourglass mode m
@@ -384,7 +420,7 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
         }
         const double volinv = 1.0 / determ;
         double VX[8], VY[8], VZ[8];
-        gather8(in.xd, nid, VX); gather8(in.yd, nid, VY); gather8(in.zd, nid, VZ);
+        gather8p(in.xd, nid, VX); gather8p(in.yd, nid, VY); gather8p(in.zd, nid, VZ);
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
           double g[8];
@@ -412,7 +448,7 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
           }
         }
         const double volume13 = cbrt_cr(determ);
-        const double coefficient = ((((-hourg) * 0.01) * f.ss[e]) * f.elemMass[e]) / volume13;
+        const double coefficient = ((((-hourg) * 0.01) * at(f.ss, eo)) * at(f.elemMass, eo)) / volume13;
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
           hgf[n][0] = coefficient * hgf[n][0];
@@ -486,35 +522,35 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
     // node plane ek is complete: calcAccelerationForNodes ..
     // calcPositionForNodes for it
     if (node_col && ek >= k0) {
-      const int64_t n = ((int64_t)ek * s1 + nj) * s1 + ni;
+      const uint32_t no = (((uint32_t)ek * (uint32_t)s1 + (uint32_t)nj) * (uint32_t)s1 + (uint32_t)ni) * 8u;
       double fx = cs0, fy = cs1, fz = cs2;
       if (hg_on) {
         fx = fx + ch0;
         fy = fy + ch1;
         fz = fz + ch2;
       }
-      const double mass = f.nodalMass[n];
+      const double mass = at(f.nodalMass, no);
       double xdd = fx / mass, ydd = fy / mass, zdd = fz / mass;
       // symmetry planes: the col 0, row 0 and plane 0 node sets
       // (src/domain.jl:773-791, 1787-1806)
       if (ni == 0) xdd = 0.0;
       if (nj == 0) ydd = 0.0;
       if (ek == 0) zdd = 0.0;
-      double xd = in.xd[n] + xdd * dt;
+      double xd = at(in.xd, no) + xdd * dt;
       if (fabs(xd) < u_cut) xd = 0.0;
-      double yd = in.yd[n] + ydd * dt;
+      double yd = at(in.yd, no) + ydd * dt;
       if (fabs(yd) < u_cut) yd = 0.0;
-      double zd = in.zd[n] + zdd * dt;
+      double zd = at(in.zd, no) + zdd * dt;
       if (fabs(zd) < u_cut) zd = 0.0;
-      out.xd[n] = xd;
-      out.yd[n] = yd;
-      out.zd[n] = zd;
-      out.x[n] = in.x[n] + xd * dt;
-      out.y[n] = in.y[n] + yd * dt;
-      out.z[n] = in.z[n] + zd * dt;
+      at(out.xd, no) = xd;
+      at(out.yd, no) = yd;
+      at(out.zd, no) = zd;
+      at(out.x, no) = at(in.x, no) + xd * dt;
+      at(out.y, no) = at(in.y, no) + yd * dt;
+      at(out.z, no) = at(in.z, no) + zd * dt;
       if (keep_aux) {
-        f.fx[n] = fx; f.fy[n] = fy; f.fz[n] = fz;
-        f.xdd[n] = xdd; f.ydd[n] = ydd; f.zdd[n] = zdd;
+        at(f.fx, no) = fx; at(f.fy, no) = fy; at(f.fz, no) = fz;
+        at(f.xdd, no) = xdd; at(f.ydd, no) = ydd; at(f.zdd, no) = zdd;
       }
     }
     cs0 = ns0; cs1 = ns1; cs2 = ns2;
@@ -539,19 +575,24 @@ __global__ __launch_bounds__(BLOCK) void k_kin(Batch B) {
   if (e >= E) return;
   if (B.gated && !sc->active) return;
   const int keep_aux = B.keep_aux;
-  int64_t nid[8];
-  elem_nodes(e, s, nid);
+  uint32_t nid[4];
+  {
+    const uint32_t ue = (uint32_t)e, us = (uint32_t)s, s1 = us + 1u;
+    const uint32_t col = ue % us, rest = ue / us;
+    hex_pairs(((rest / us) * s1 + rest % us) * s1 + col, s1, nid);
+  }
+  const uint32_t eo = (uint32_t)e * 8u;
   double X[8], Y[8], Z[8], XV[8], YV[8], ZV[8];
   const NodeSet S = pick(f, sc->parity);
-  gather8(S.x, nid, X); gather8(S.y, nid, Y); gather8(S.z, nid, Z);
-  gather8(S.xd, nid, XV); gather8(S.yd, nid, YV); gather8(S.zd, nid, ZV);
-  const double volo = f.volo[e];
+  gather8p(S.x, nid, X); gather8p(S.y, nid, Y); gather8p(S.z, nid, Z);
+  gather8p(S.xd, nid, XV); gather8p(S.yd, nid, YV); gather8p(S.zd, nid, ZV);
+  const double volo = at(f.volo, eo);
   const double volume = elem_volume(X, Y, Z);
   const double vnew = volume / volo;
   if (vnew <= 0.0) raise(sc, ERR_KIN_VOLUME);
-  f.vnew[e] = vnew;
-  f.delv[e] = vnew - f.v[e];
-  f.arealg[e] = char_length(X, Y, Z, volume);
+  at(f.vnew, eo) = vnew;
+  at(f.delv, eo) = vnew - at(f.v, eo);
+  at(f.arealg, eo) = char_length(X, Y, Z, volume);
   {
     const double dt2 = 0.5 * sc->deltatime;
     double XS[8], YS[8], ZS[8], b[4][3];
@@ -567,12 +608,12 @@ __global__ __launch_bounds__(BLOCK) void k_kin(Batch B) {
     const double d2 = inv_detJ * vgrad_term(b, 1, YV);
     const double d3 = inv_detJ * vgrad_term(b, 2, ZV);
     const double vdov = (d1 + d2) + d3;
-    f.vdov[e] = vdov;
+    at(f.vdov, eo) = vdov;
     if (keep_aux) {
       const double vdovthird = vdov / 3.0;
-      f.dxx[e] = d1 - vdovthird;
-      f.dyy[e] = d2 - vdovthird;
-      f.dzz[e] = d3 - vdovthird;
+      at(f.dxx, eo) = d1 - vdovthird;
+      at(f.dyy, eo) = d2 - vdovthird;
+      at(f.dzz, eo) = d3 - vdovthird;
     }
     if (vnew <= 0.0) raise(sc, ERR_LAG_VOLUME);
   }
@@ -594,32 +635,32 @@ __global__ __launch_bounds__(BLOCK) void k_kin(Batch B) {
   ax = (dyi * dzj) - (dzi * dyj);
   ay = (dzi * dxj) - (dxi * dzj);
   az = (dxi * dyj) - (dyi * dxj);
-  f.delx_zeta[e] = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  at(f.delx_zeta, eo) = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
   ax = ax * norm; ay = ay * norm; az = az * norm;
   dxv = 0.25 * (sum4(XV[4], XV[5], XV[6], XV[7]) - sum4(XV[0], XV[1], XV[2], XV[3]));
   dyv = 0.25 * (sum4(YV[4], YV[5], YV[6], YV[7]) - sum4(YV[0], YV[1], YV[2], YV[3]));
   dzv = 0.25 * (sum4(ZV[4], ZV[5], ZV[6], ZV[7]) - sum4(ZV[0], ZV[1], ZV[2], ZV[3]));
-  f.delv_zeta[e] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+  at(f.delv_zeta, eo) = ((ax * dxv) + (ay * dyv)) + (az * dzv);
   // xi: j cross k
   ax = (dyj * dzk) - (dzj * dyk);
   ay = (dzj * dxk) - (dxj * dzk);
   az = (dxj * dyk) - (dyj * dxk);
-  f.delx_xi[e] = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  at(f.delx_xi, eo) = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
   ax = ax * norm; ay = ay * norm; az = az * norm;
   dxv = 0.25 * (sum4(XV[1], XV[2], XV[6], XV[5]) - sum4(XV[0], XV[3], XV[7], XV[4]));
   dyv = 0.25 * (sum4(YV[1], YV[2], YV[6], YV[5]) - sum4(YV[0], YV[3], YV[7], YV[4]));
   dzv = 0.25 * (sum4(ZV[1], ZV[2], ZV[6], ZV[5]) - sum4(ZV[0], ZV[3], ZV[7], ZV[4]));
-  f.delv_xi[e] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+  at(f.delv_xi, eo) = ((ax * dxv) + (ay * dyv)) + (az * dzv);
   // eta: k cross i
   ax = (dyk * dzi) - (dzk * dyi);
   ay = (dzk * dxi) - (dxk * dzi);
   az = (dxk * dyi) - (dyk * dxi);
-  f.delx_eta[e] = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  at(f.delx_eta, eo) = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
   ax = ax * norm; ay = ay * norm; az = az * norm;
   dxv = -0.25 * (sum4(XV[0], XV[1], XV[5], XV[4]) - sum4(XV[3], XV[2], XV[6], XV[7]));
   dyv = -0.25 * (sum4(YV[0], YV[1], YV[5], YV[4]) - sum4(YV[3], YV[2], YV[6], YV[7]));
   dzv = -0.25 * (sum4(ZV[0], ZV[1], ZV[5], ZV[4]) - sum4(ZV[3], ZV[2], ZV[6], ZV[7]));
-  f.delv_eta[e] = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+  at(f.delv_eta, eo) = ((ax * dxv) + (ay * dyv)) + (az * dzv);
 }
 
 // neighbour ids exactly as setupElementConnectivities stores them
