@@ -38,7 +38,6 @@ def alg_bytes(E, N):
         # read, e p q ss v 40 written
         "k_qeos": 172 * E,
         "k_dt": 16 * ((E + 255) // 256),
-        "k_flip": 0,
     }
 
 

@@ -195,8 +195,7 @@ int lulesh_shadow_download(lulesh_ctx *ctx, const char *field, double *dst, int6
 #define LULESH_K_KIN 1   /* kinematics + MonoQ gradients */
 #define LULESH_K_QEOS 2  /* MonoQ limiter, EOS, volume update, dt partial minima */
 #define LULESH_K_DT 3    /* dt minima + next timeIncrement! */
-#define LULESH_K_FLIP 4  /* makes the positions k_nodal wrote current */
-#define LULESH_K_COUNT 5
+#define LULESH_K_COUNT 4
 int lulesh_set_timing(lulesh_ctx *ctx, int32_t on);
 /* k_nodal geometry (no reference counterpart): each workgroup marches
  * `depth` node planes of its 15x15 node column.  0 restores the default

@@ -1,12 +1,11 @@
 // Adjoint kernels of one LULESH cycle on gfx950 (reverse of k_nodal ->
-// k_flip -> k_kin -> k_qeos with the cycle's dt held fixed).
+// k_kin -> k_qeos with the cycle's dt held fixed).
 //
-// The reverse of cycle c runs after the forward cycle has been recomputed
-// from its stored start state S_c = (x y z xd yd zd, e p q v ss) through
-// k_nodal, k_flip and k_kin, so the kernels below see S_c in the element
-// arrays and the old node set, and the cycle's x', xd', vnew, delv, delx_*,
-// delv_* in the current node set / temporaries.  On entry the adjoint
-// buffers hold the adjoint of S_{c+1}; on exit that of S_c:
+// The kernels read cycle c from a stored slot (AdjArgs): its start state
+// S_c = (x y z xd yd zd, e p q v ss), the node set after it (x', xd') and its
+// temporaries (vnew delv vdov delx_* delv_*), all kept when the forward
+// cycle ran.  On entry the adjoint buffers hold the adjoint of S_{c+1}; on
+// exit that of S_c:
 //   a_qeos  EOS + volume update + MonoQ region (limiter) adjoints
 //   a_kin   MonoQ gradients + relative volume adjoints -> corners (x', xd')
 //   a_node1 corner gather + node-update adjoint -> force adjoint per node
@@ -218,7 +217,7 @@ __global__ __launch_bounds__(BLOCK) void a_kin(AdjArgs A) {
 LDEV int corner_of_a(int ox, int oy, int oz) { return 4 * oz + (oy ? (ox ? 2 : 3) : (ox ? 1 : 0)); }
 
 // sum over the node's corners of comp c (0..5) of the corner buffer
-LDEV void gather_corners(const AdjArgs &A, int64_t n, double out[6]) {
+LDEV void gather_corners(const AdjArgs &A, const double *corner, int64_t n, double out[6]) {
   const int s = A.s;
   const int64_t s1 = s + 1;
   const int i = (int)(n % s1), j = (int)((n / s1) % s1), k = (int)(n / (s1 * s1));
@@ -234,7 +233,7 @@ LDEV void gather_corners(const AdjArgs &A, int64_t n, double out[6]) {
         if (ei < 0 || ei >= s) continue;
         const int64_t e = ((int64_t)ek * s + ej) * s + ei;
         const int cn = corner_of_a(-di, -dj, -dk);
-        for (int c = 0; c < 6; ++c) out[c] += A.corner[(int64_t)(c * 8 + cn) * A.E + e];
+        for (int c = 0; c < 6; ++c) out[c] += corner[(int64_t)(c * 8 + cn) * A.E + e];
       }
     }
   }
@@ -245,7 +244,13 @@ __global__ __launch_bounds__(BLOCK) void a_node1(AdjArgs A) {
   const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (n >= A.N) return;
   double g[6];
-  gather_corners(A, n, g);
+  if (A.force_pending) {
+    // what a_node2 of the later cycle would add: corner_f into the node state
+    gather_corners(A, A.corner_f, n, g);
+    A.lx[n] += g[0]; A.ly[n] += g[1]; A.lz[n] += g[2];
+    A.lxd[n] += g[3]; A.lyd[n] += g[4]; A.lzd[n] += g[5];
+  }
+  gather_corners(A, A.corner, n, g);
   const double dt = A.dt;
   const int64_t s1 = A.s + 1;
   const int i = (int)(n % s1), j = (int)((n / s1) % s1), k = (int)(n / (s1 * s1));
@@ -301,7 +306,7 @@ __global__ __launch_bounds__(BLOCK) void a_force(AdjArgs A) {
     X[n] = S.x[nid[n]]; Y[n] = S.y[nid[n]]; Z[n] = S.z[nid[n]];
   }
   const NodeSrc src{S.xd, S.yd, S.zd, A.lF, A.N, nid};
-  CornerSink sink{A.corner, A.E, e};
+  CornerSink sink{A.corner_f, A.E, e};
   double lX[8] = {0}, lY[8] = {0}, lZ[8] = {0};
   const double hourg = A.sc->hgcoef;
   if (hourg > 0.0) {
@@ -321,7 +326,7 @@ __global__ __launch_bounds__(BLOCK) void a_force(AdjArgs A) {
   const double lms = lulesh_adj::stress_rev_s(X, Y, Z, Q1, ms, src, lX, lY, lZ);
   A.lp[e] += lms;
   A.lq[e] += lms;
-  double *cr = A.corner;
+  double *cr = A.corner_f;
   for (int n = 0; n < 8; ++n) {
     cr[(int64_t)(0 * 8 + n) * A.E + e] = lX[n];
     cr[(int64_t)(1 * 8 + n) * A.E + e] = lY[n];
@@ -333,29 +338,13 @@ __global__ __launch_bounds__(BLOCK) void a_node2(AdjArgs A) {
   const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (n >= A.N) return;
   double g[6];
-  gather_corners(A, n, g);
+  gather_corners(A, A.corner_f, n, g);
   A.lx[n] += g[0];
   A.ly[n] += g[1];
   A.lz[n] += g[2];
   A.lxd[n] += g[3];
   A.lyd[n] += g[4];
   A.lzd[n] += g[5];
-}
-
-// replayed time step of cycle `idx` of a recorded schedule: the time and
-// cycle bookkeeping of timeIncrement! with dt taken from the schedule
-__global__ void a_set_dt(DevScalars *sc, const double *dt, int64_t base) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double d = dt[sc->cycle - base];
-  sc->deltatime = d;
-  sc->time = sc->time + d;
-  sc->cycle = sc->cycle + 1;
-}
-
-// deltatime of the cycle just incremented, into the schedule
-__global__ void a_record_dt(DevScalars *sc, double *dt, int64_t base) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  dt[sc->cycle - 1 - base] = sc->deltatime;
 }
 
 // loss seed: dL/de = elemMass (L = sum e * elemMass), everything else 0
@@ -394,19 +383,34 @@ __global__ __launch_bounds__(BLOCK) void a_state_copy(DevScalars *sc, DevFields 
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void a_store_temps(DevScalars *sc, DevFields f, int64_t E,
-                                                       int64_t N, double *buf) {
+__global__ __launch_bounds__(BLOCK) void a_slot_step(DevScalars *sc, DevFields f, int64_t E,
+                                                     int64_t N, double *temps, double *state,
+                                                     const double *dt, int64_t base) {
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   const int par = sc->parity;
-  const double *nd[6] = {par ? f.ns[1].x : f.ns[0].x, par ? f.ns[1].y : f.ns[0].y,
-                         par ? f.ns[1].z : f.ns[0].z, par ? f.ns[1].xd : f.ns[0].xd,
-                         par ? f.ns[1].yd : f.ns[0].yd, par ? f.ns[1].zd : f.ns[0].zd};
-  const double *el[9] = {f.vnew, f.delv, f.vdov, f.delx_xi, f.delx_eta, f.delx_zeta,
-                         f.delv_xi, f.delv_eta, f.delv_zeta};
-  if (i < N)
-    for (int k = 0; k < 6; ++k) buf[k * N + i] = nd[k][i];
-  if (i < E)
-    for (int k = 0; k < 9; ++k) buf[6 * N + k * E + i] = el[k][i];
+  if (temps && i < E) {
+    const double *el[9] = {f.vnew, f.delv, f.vdov, f.delx_xi, f.delx_eta, f.delx_zeta,
+                           f.delv_xi, f.delv_eta, f.delv_zeta};
+    for (int k = 0; k < 9; ++k) temps[k * E + i] = el[k][i];
+  }
+  if (state) {
+    if (i < N) {
+      const double *nd[6] = {par ? f.ns[1].x : f.ns[0].x, par ? f.ns[1].y : f.ns[0].y,
+                             par ? f.ns[1].z : f.ns[0].z, par ? f.ns[1].xd : f.ns[0].xd,
+                             par ? f.ns[1].yd : f.ns[0].yd, par ? f.ns[1].zd : f.ns[0].zd};
+      for (int k = 0; k < 6; ++k) state[k * N + i] = nd[k][i];
+    }
+    if (i < E) {
+      const double *el[5] = {f.e, f.p, f.q, f.v, f.ss};
+      for (int k = 0; k < 5; ++k) state[6 * N + k * E + i] = el[k][i];
+    }
+  }
+  if (dt && i == 0) {
+    const double d = dt[sc->cycle - base];
+    sc->deltatime = d;
+    sc->time = sc->time + d;
+    sc->cycle = sc->cycle + 1;
+  }
 }
 
 __global__ void a_set_cycle(DevScalars *sc, long long cycle) {
@@ -455,25 +459,16 @@ hipError_t launch_state_copy(hipStream_t st, DevScalars *sc, const DevFields &f,
   return hipGetLastError();
 }
 
-hipError_t launch_store_temps(hipStream_t st, DevScalars *sc, const DevFields &f, int64_t E,
-                              int64_t N, double *buf) {
-  hipLaunchKernelGGL(a_store_temps, dim3(nblk_a(N > E ? N : E)), dim3(BLOCK), 0, st, sc, f, E, N,
-                     buf);
+hipError_t launch_slot_step(hipStream_t st, DevScalars *sc, const DevFields &f, int64_t E,
+                            int64_t N, double *temps, double *state, const double *dt,
+                            int64_t base) {
+  hipLaunchKernelGGL(a_slot_step, dim3(nblk_a(N > E ? N : E)), dim3(BLOCK), 0, st, sc, f, E, N,
+                     temps, state, dt, base);
   return hipGetLastError();
 }
 
 hipError_t launch_set_cycle(hipStream_t st, DevScalars *sc, long long cycle) {
   hipLaunchKernelGGL(a_set_cycle, dim3(1), dim3(64), 0, st, sc, cycle);
-  return hipGetLastError();
-}
-
-hipError_t launch_set_dt(hipStream_t st, DevScalars *sc, const double *dt, int64_t base) {
-  hipLaunchKernelGGL(a_set_dt, dim3(1), dim3(64), 0, st, sc, dt, base);
-  return hipGetLastError();
-}
-
-hipError_t launch_record_dt(hipStream_t st, DevScalars *sc, double *dt, int64_t base) {
-  hipLaunchKernelGGL(a_record_dt, dim3(1), dim3(64), 0, st, sc, dt, base);
   return hipGetLastError();
 }
 

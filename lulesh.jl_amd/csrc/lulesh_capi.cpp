@@ -100,6 +100,7 @@ Batch make_args(lulesh_ctx *c, int gated, int loop_mode, int64_t num_iters = 0) 
   a.loop_mode = loop_mode;
   a.keep_aux = c->keep_aux;
   a.num_iters = num_iters;
+  a.flip = 1;
   return a;
 }
 
@@ -241,7 +242,7 @@ int build_graph(lulesh_ctx *c) {
   const Batch a = make_args(c, 1, 1);
   hipGraph_t g = nullptr;
   HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
+  const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
   for (int id : ids) {
     hipError_t e = launch_kernel(id, c->stream, a);
     if (e != hipSuccess) {
@@ -265,12 +266,23 @@ int build_graph(lulesh_ctx *c) {
 // cz <= 0 picks the default: the deepest of 16, 8, 4, 2 that still gives
 // the launch >= 1024 workgroups (small domains trade some recomputed halo
 // planes for parallelism).  Uploads the instance and block tables.
+// z-march depth of an instance: `depth` node planes per chunk, balanced so
+// that the chunks of a column are (nearly) equal
+int balanced_depth(int s, int depth) {
+  const int m = s + 1, chunks = (m + depth - 1) / depth;
+  return (m + chunks - 1) / chunks;
+}
+
 int plan_nodal(lulesh_ctx *c, int cz) {
-  if (cz <= 0) {
+  // default: the deepest of 16/8/4/2 that still launches >= 1024 workgroups,
+  // balanced per instance; an explicit depth applies unchanged to all
+  const bool automatic = cz <= 0;
+  if (automatic) {
     cz = 16;
     for (;;) {
       int64_t total = 0;
-      for (const InstDesc &I : c->inst) total += nodal_blocks(I.s, cz);
+      for (const InstDesc &I : c->inst)
+        total += nodal_blocks(NodalTile{I.etx, I.ety, I.tx, I.ty}, I.s, balanced_depth(I.s, cz));
       if (total >= 1024 || cz <= 2) break;
       cz /= 2;
     }
@@ -279,9 +291,9 @@ int plan_nodal(lulesh_ctx *c, int cz) {
   int gb = 0;
   for (int i = 0; i < c->n; ++i) {
     InstDesc &I = c->inst[i];
-    I.cz = cz;
+    I.cz = automatic ? balanced_depth(I.s, cz) : cz;
     I.gb0 = gb;
-    I.gbn = nodal_blocks(I.s, cz);
+    I.gbn = nodal_blocks(NodalTile{I.etx, I.ety, I.tx, I.ty}, I.s, I.cz);
     gb += I.gbn;
     gblock_inst.insert(gblock_inst.end(), I.gbn, i);
   }
@@ -329,7 +341,11 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
     InstDesc &I = c->inst[i];
     const int s = nx[i];
     I.s = s;
-    I.tiles = nodal_tiles(s);
+    const NodalTile tile = nodal_tile(s);
+    I.etx = tile.etx;
+    I.ety = tile.ety;
+    I.tx = tile.tx;
+    I.ty = tile.ty;
     I.E = (int64_t)s * s * s;
     I.N = (int64_t)(s + 1) * (s + 1) * (s + 1);
     I.e0 = e0;
@@ -347,7 +363,7 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
     n0 += I.N;
     eb += I.ebn;
     nb += I.nbn;
-    gb_max += nodal_blocks(s, 1);
+    gb_max += nodal_blocks(tile, s, 1);
     eblock_inst.insert(eblock_inst.end(), I.ebn, i);
     nblock_inst.insert(nblock_inst.end(), I.nbn, i);
     c->nr.push_back(nr ? nr[i] : 1);
@@ -456,7 +472,7 @@ int run_cycles(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run /* [n] or n
       if (use_graph) {
         HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
       } else {
-        const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
+        const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
         for (int id : ids) {
           rc = launch(c, id, a);
           if (rc) return rc;
@@ -503,10 +519,11 @@ int adj_alloc(lulesh_ctx *c, double **p, int64_t n) {
 }
 
 // one cycle of a segment: its start state S_k (6N + 5E, the checkpoint
-// layout), the node set after it (6N) and its temporaries (9E)
+// layout) and its temporaries (9E); a segment of L cycles uses L + 1 slots,
+// the last holding only the state after it
 int64_t cycle_slot(const lulesh_ctx *c) {
   const int64_t E = c->inst[0].E, N = c->inst[0].N;
-  return 12 * N + 14 * E;
+  return 6 * N + 14 * E;
 }
 
 // dt schedule of `ncycles`; with `every` > 0 also the adjoint fields, the
@@ -543,6 +560,7 @@ int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
     if (!rc) rc = adj_alloc(c, &a.ldvs, 3 * E);
     if (!rc) rc = adj_alloc(c, &a.ldx, 3 * E);
     if (!rc) rc = adj_alloc(c, &a.corner, 48 * E);
+    if (!rc) rc = adj_alloc(c, &a.corner_f, 48 * E);
     if (!rc) rc = adj_alloc(c, &a.pacc, 3 * E);
     if (!rc) rc = adj_alloc(c, &B.final_state, state);
     if (!rc) {
@@ -565,7 +583,7 @@ int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
     B.ckpt_cap = nck;
   }
   if (every > B.seg_cap) {
-    rc = adj_alloc(c, &B.seg, every * cycle_slot(c));
+    rc = adj_alloc(c, &B.seg, (every + 1) * cycle_slot(c));
     if (rc) return rc;
     B.seg_cap = every;
   }
@@ -578,12 +596,24 @@ int state_copy(lulesh_ctx *c, double *buf, int to_buf) {
   return e == hipSuccess ? 0 : hip_fail(c, e, "state copy");
 }
 
-// forward cycle with dt from the recorded schedule (timeIncrement! replaced)
-int replay_cycle(lulesh_ctx *c, const double *dt, int64_t base, bool constraints) {
+// temporaries of the cycle just run into `temps`, the current state into
+// `state` and the next cycle's scheduled dt (each step optional; one launch)
+int slot_step(lulesh_ctx *c, double *temps, double *state, const double *dt, int64_t base) {
+  const int64_t E = c->inst[0].E, N = c->inst[0].N;
+  hipError_t e = launch_slot_step(c->stream, c->b.sc, c->b.f, E, N, temps, state, dt, base);
+  return e == hipSuccess ? 0 : hip_fail(c, e, "slot step");
+}
+
+// slot j of the segment buffer; its temporaries follow the 6N + 5E state
+double *seg_slot(lulesh_ctx *c, int64_t j) { return c->adj.seg + j * cycle_slot(c); }
+double *slot_temps(lulesh_ctx *c, double *slot) {
+  return slot + 6 * c->inst[0].N + 5 * c->inst[0].E;
+}
+
+// the cycle after the scheduled dt was set (timeIncrement! replaced)
+int cycle_body(lulesh_ctx *c, bool constraints) {
   const Batch a = make_args(c, 0, 0);
-  hipError_t e = launch_set_dt(c->stream, c->b.sc, dt, base);
-  if (e != hipSuccess) return hip_fail(c, e, "set dt");
-  const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
+  const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
   for (int id : ids) {
     if (id == KID_DT && !constraints) break;
     int rc = launch(c, id, a);
@@ -592,38 +622,37 @@ int replay_cycle(lulesh_ctx *c, const double *dt, int64_t base, bool constraints
   return 0;
 }
 
-// adjoint kernel arguments reading cycle k from its stored slot
-AdjArgs stored_args(lulesh_ctx *c, const double *slot, double dt) {
+// adjoint kernel arguments reading cycle k from its stored slot (start
+// state + temporaries) and the next slot (node set after the cycle)
+AdjArgs stored_args(lulesh_ctx *c, double *slot, double *next, double dt, int force_pending) {
   const int64_t E = c->inst[0].E, N = c->inst[0].N;
   AdjArgs a = c->adj.a;
-  double *p = const_cast<double *>(slot);
+  double *p = slot;
   a.old = NodeSet{p, p + N, p + 2 * N, p + 3 * N, p + 4 * N, p + 5 * N};
   double *el = p + 6 * N;
   a.f.e = el; a.f.p = el + E; a.f.q = el + 2 * E; a.f.v = el + 3 * E; a.f.ss = el + 4 * E;
-  double *nw = el + 5 * E;
-  a.cur = NodeSet{nw, nw + N, nw + 2 * N, nw + 3 * N, nw + 4 * N, nw + 5 * N};
-  double *t = nw + 6 * N;
+  double *t = el + 5 * E;
   a.f.vnew = t; a.f.delv = t + E; a.f.vdov = t + 2 * E;
   a.f.delx_xi = t + 3 * E; a.f.delx_eta = t + 4 * E; a.f.delx_zeta = t + 5 * E;
   a.f.delv_xi = t + 6 * E; a.f.delv_eta = t + 7 * E; a.f.delv_zeta = t + 8 * E;
+  double *nw = next;
+  a.cur = NodeSet{nw, nw + N, nw + 2 * N, nw + 3 * N, nw + 4 * N, nw + 5 * N};
   a.dt = dt;
+  a.force_pending = force_pending;
   return a;
 }
 
-// the five adjoint kernels of one cycle
-int adjoint_cycle(lulesh_ctx *c, const AdjArgs &aa) {
+// one cycle's adjoint kernels; a_node1 first folds in the force corners
+// the later cycle's a_force left (force_pending), so the node-state gather
+// of a_force runs as a separate a_node2 only after the last cycle
+int adjoint_cycle(lulesh_ctx *c, const AdjArgs &aa, bool last) {
   const int ks[] = {AK_QEOS, AK_KIN, AK_NODE1, AK_FORCE, AK_NODE2};
   for (int kk : ks) {
+    if (kk == AK_NODE2 && !last) break;
     hipError_t e = launch_adjoint(kk, c->stream, aa);
     if (e != hipSuccess) return hip_fail(c, e, "adjoint kernel");
   }
   return 0;
-}
-
-int store_temps(lulesh_ctx *c, double *slot) {
-  const int64_t E = c->inst[0].E, N = c->inst[0].N;
-  hipError_t e = launch_store_temps(c->stream, c->b.sc, c->b.f, E, N, slot + 6 * N + 5 * E);
-  return e == hipSuccess ? 0 : hip_fail(c, e, "store temporaries");
 }
 
 int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, double grad_params[3],
@@ -640,16 +669,15 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
   if (rc) return rc;
   const long long cycle0 = c->hsc[0].cycle;
   const Batch a = make_args(c, 0, 0);
+  Batch ta = a;  // timeIncrement! that also records the dt it takes
+  ta.dt_record = B.dt;
+  ta.dt_base = cycle0;
   // 1. forward pass: checkpoints every `every` cycles, dt schedule recorded
   rc = clear_err(c);
   for (int64_t k = 0; k < ncycles && !rc; ++k) {
     if (k % every == 0) rc = state_copy(c, B.ckpt + (k / every) * state, 1);
-    if (!rc) rc = launch(c, KID_TIME_INCREMENT, a);
-    if (!rc) {
-      hipError_t e = launch_record_dt(c->stream, c->b.sc, B.dt, cycle0);
-      if (e != hipSuccess) rc = hip_fail(c, e, "record dt");
-    }
-    const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
+    if (!rc) rc = launch(c, KID_TIME_INCREMENT, ta);
+    const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
     for (int id : ids)
       if (!rc) rc = launch(c, id, a);
   }
@@ -677,23 +705,28 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
     hipError_t e = launch_adjoint(AK_SEED, c->stream, B.a);
     if (e != hipSuccess) return hip_fail(c, e, "adjoint seed");
   }
-  const int64_t slot = cycle_slot(c);
   const int64_t nseg = (ncycles + every - 1) / every;
   for (int64_t sg = nseg - 1; sg >= 0 && !rc; --sg) {
     const int64_t k0 = sg * every, k1 = (k0 + every < ncycles) ? k0 + every : ncycles;
+    const int64_t L = k1 - k0;
     rc = state_copy(c, B.ckpt + sg * state, 0);
     if (!rc) {
       hipError_t e = launch_set_cycle(c->stream, c->b.sc, cycle0 + k0);
       if (e != hipSuccess) rc = hip_fail(c, e, "set cycle");
     }
-    for (int64_t k = k0; k < k1 && !rc; ++k) {
-      double *sl = B.seg + (k - k0) * slot;
-      rc = state_copy(c, sl, 1);
-      if (!rc) rc = replay_cycle(c, B.dt, cycle0, false);
-      if (!rc) rc = store_temps(c, sl);
+    // slot j <- S_{k0+j} (and the previous cycle's temporaries), dt, cycle
+    for (int64_t j = 0; j < L && !rc; ++j) {
+      rc = slot_step(c, j ? slot_temps(c, seg_slot(c, j - 1)) : nullptr, seg_slot(c, j), B.dt,
+                     cycle0);
+      if (!rc) rc = cycle_body(c, false);
     }
-    for (int64_t k = k1 - 1; k >= k0 && !rc; --k)
-      rc = adjoint_cycle(c, stored_args(c, B.seg + (k - k0) * slot, hdt[k]));
+    if (!rc) rc = slot_step(c, slot_temps(c, seg_slot(c, L - 1)), seg_slot(c, L), nullptr, 0);
+    for (int64_t j = L - 1; j >= 0 && !rc; --j) {
+      const int64_t k = k0 + j;
+      rc = adjoint_cycle(
+          c, stored_args(c, seg_slot(c, j), seg_slot(c, j + 1), hdt[k], k == ncycles - 1 ? 0 : 1),
+          k == 0);
+    }
   }
   if (rc) return rc;
   // 3. results, then the state the forward pass ended in
@@ -726,18 +759,20 @@ int vjp_cycle(lulesh_ctx *c, double grad_params[3]) {
   const int64_t E = c->inst[0].E;
   const Batch a = make_args(c, 0, 0);
   // the forward cycle exactly as lulesh_lagrange_leapfrog runs it, with its
-  // start state and temporaries kept in a cycle slot
-  rc = state_copy(c, B.seg, 1);
+  // start state and temporaries kept in slot 0 and the state after it in
+  // slot 1
+  rc = slot_step(c, nullptr, seg_slot(c, 0), nullptr, 0);
   if (!rc) rc = clear_err(c);
-  const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
+  const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
   for (int id : ids)
     if (!rc) rc = launch(c, id, a);
-  if (!rc) rc = store_temps(c, B.seg);
+  if (!rc) rc = slot_step(c, slot_temps(c, seg_slot(c, 0)), seg_slot(c, 1), nullptr, 0);
   if (!rc) rc = finish(c, true);
   if (rc) return rc;
   c->partials_valid = false;
   HIP_TRY(c, hipMemsetAsync(B.a.pacc, 0, sizeof(double) * 3 * E, c->stream));
-  rc = adjoint_cycle(c, stored_args(c, B.seg, c->hsc[0].deltatime));
+  rc = adjoint_cycle(c, stored_args(c, seg_slot(c, 0), seg_slot(c, 1), c->hsc[0].deltatime, 0),
+                     true);
   if (rc) return rc;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (grad_params) {
@@ -825,7 +860,10 @@ int lulesh_replay(lulesh_ctx *c, const double *dt, int64_t ncycles) {
   if (rc) return rc;
   const long long cycle0 = c->hsc[0].cycle;
   rc = clear_err(c);
-  for (int64_t k = 0; k < ncycles && !rc; ++k) rc = replay_cycle(c, c->adj.dt, cycle0, true);
+  for (int64_t k = 0; k < ncycles && !rc; ++k) {
+    rc = slot_step(c, nullptr, nullptr, c->adj.dt, cycle0);
+    if (!rc) rc = cycle_body(c, true);
+  }
   if (rc) return rc;
   c->partials_valid = false;
   return finish(c, true);
@@ -1013,7 +1051,6 @@ int lulesh_lagrange_nodal(lulesh_ctx *c) {
   const Batch a = make_args(c, 0, 0);
   int rc = clear_err(c);
   if (!rc) rc = launch(c, KID_NODAL, a);
-  if (!rc) rc = launch(c, KID_FLIP, a);
   if (rc) return rc;
   c->partials_valid = false;
   return finish(c, true);
@@ -1049,7 +1086,7 @@ int lulesh_lagrange_leapfrog(lulesh_ctx *c) {
   HIP_TRY(c, hipSetDevice(c->device));
   const Batch a = make_args(c, 0, 0);
   int rc = clear_err(c);
-  const int ids[] = {KID_NODAL, KID_FLIP, KID_KIN, KID_QEOS, KID_DT};
+  const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
   for (int id : ids)
     if (!rc) rc = launch(c, id, a);
   if (rc) return rc;
@@ -1155,9 +1192,9 @@ int32_t lulesh_nodal_depth(const lulesh_ctx *c) { return c ? c->inst[0].cz : -1;
 int lulesh_autotune(lulesh_ctx *c, int32_t *chosen, double *best_ms) {
   if (!c) return LULESH_ERR_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
-  // k_nodal alone reads the current node set and writes only the other one
-  // (the state becomes current only through k_flip), so it can be timed on
-  // the live state without changing it
+  // k_nodal with its parity flip switched off reads the current node set
+  // and writes only the other one, so it can be timed on the live state
+  // without changing it
   const int cand[] = {2, 4, 8, 16, 32};
   int best = -1;
   float best_t = 0.f;
@@ -1170,7 +1207,8 @@ int lulesh_autotune(lulesh_ctx *c, int32_t *chosen, double *best_ms) {
     if (cz > 2 && cz / 2 > maxs + 1) break;
     rc = plan_nodal(c, cz);
     if (rc) break;
-    const Batch a = make_args(c, 0, 0);
+    Batch a = make_args(c, 0, 0);
+    a.flip = 0;
     hipError_t e = hipSuccess;
     for (int w = 0; w < 2 && e == hipSuccess; ++w) e = launch_kernel(KID_NODAL, c->stream, a);
     if (e == hipSuccess) e = hipEventRecord(ea, c->stream);
@@ -1201,7 +1239,7 @@ int lulesh_autotune(lulesh_ctx *c, int32_t *chosen, double *best_ms) {
 }
 
 const char *lulesh_kernel_name(int32_t id) {
-  static const char *names[] = {"k_nodal", "k_kin", "k_qeos", "k_dt", "k_flip"};
+  static const char *names[] = {"k_nodal", "k_kin", "k_qeos", "k_dt"};
   if (id < 0 || id >= LULESH_K_COUNT) return "";
   return names[id];
 }

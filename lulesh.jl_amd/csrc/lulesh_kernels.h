@@ -25,7 +25,8 @@ struct DevScalars {
   int err;              // lowest error code raised (ERR_NONE when clean)
   int active;           // 1 while the current cycle executes (driver loop)
   int parity;           // which NodeSet holds the current x..zd
-  int pad_;
+  int nodal_done;       // k_nodal workgroups of this instance finished in the
+                        // running launch (the last one flips parity, resets it)
 };
 
 // node positions and velocities; two sets, ping-ponged by the fused nodal
@@ -50,7 +51,8 @@ struct DevFields {
 // one instance of a (ragged) batch
 struct InstDesc {
   int s;                 // elements per edge
-  int tiles;             // k_nodal tiles per edge
+  int etx, ety;          // k_nodal tile: etx x ety elements around (etx-1) x (ety-1) nodes
+  int tx, ty;            // k_nodal tiles along x and y
   int64_t e0, n0;        // offsets into the flat arrays
   int64_t E, N;          // sizes
   int eb0, ebn;          // element-kernel blocks [eb0, eb0+ebn)
@@ -75,6 +77,9 @@ struct Batch {
   int gated;                // kernels skip instances whose active flag is 0
   int keep_aux;             // also write fx.., xdd.., dxx.., ql, qq
   int loop_mode;            // k_dt evaluates the driver-loop test and the next dt
+  int flip;                 // k_nodal makes the node set it wrote current
+  double *dt_record;        // k_time_increment stores each instance-0 dt here
+  int64_t dt_base;          // ... at index cycle - 1 - dt_base (if non-null)
   int64_t num_iters;
 };
 
@@ -83,22 +88,38 @@ enum KernelId {
   KID_KIN = 1,
   KID_QEOS = 2,
   KID_DT = 3,
-  KID_FLIP = 4,
-  KID_DTPART = 5,
-  KID_TIME_INCREMENT = 6,
-  KID_RUN_BEGIN = 7,
-  KID_CLEAR_ERR = 8,
+  KID_DTPART = 4,
+  KID_TIME_INCREMENT = 5,
+  KID_RUN_BEGIN = 6,
+  KID_CLEAR_ERR = 7,
 };
 
 constexpr int BLOCK = 256;
-// k_nodal tiling: 16x16 element columns around 15x15 node columns; each
-// workgroup marches `cz` node planes (chosen per context, see plan_nodal)
-constexpr int NODAL_NT = 15;
-
-inline int nodal_tiles(int s) { return (s + 1 + NODAL_NT - 1) / NODAL_NT; }
-inline int nodal_blocks(int s, int cz) {
-  const int t = nodal_tiles(s);
-  return t * t * ((s + 1 + cz - 1) / cz);
+// k_nodal tiling: a workgroup owns an (etx-1) x (ety-1) column of nodes and
+// computes the etx x ety elements around it (etx * ety <= BLOCK), marching
+// `cz` node planes up the column.  Per instance the tile is the one that
+// covers the (s+1)^2 node columns with the fewest tiles (16 x 16 elements
+// for large s; the squarest among equals), so small domains do not leave
+// most of a 16 x 16 tile idle.
+struct NodalTile {
+  int etx, ety, tx, ty;
+};
+inline NodalTile nodal_tile(int s) {
+  const int m = s + 1;
+  NodalTile best{16, 16, (m + 14) / 15, (m + 14) / 15};
+  for (int a = 1; a + 1 <= BLOCK; ++a) {
+    const int b = BLOCK / (a + 1) - 1;
+    if (b < 1) break;
+    const int tx = (m + a - 1) / a, ty = (m + b - 1) / b;
+    const int cur = best.tx * best.ty, cand = tx * ty;
+    const int dcur = best.etx > best.ety ? best.etx - best.ety : best.ety - best.etx;
+    const int dcand = a > b ? a - b : b - a;
+    if (cand < cur || (cand == cur && dcand < dcur)) best = NodalTile{a + 1, b + 1, tx, ty};
+  }
+  return best;
+}
+inline int nodal_blocks(const NodalTile &t, int s, int cz) {
+  return t.tx * t.ty * ((s + 1 + cz - 1) / cz);
 }
 
 hipError_t launch_init(hipStream_t st, const Batch &b);

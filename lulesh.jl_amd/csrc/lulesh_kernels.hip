@@ -282,6 +282,7 @@ __global__ void k_init_scalars(Batch B) {
   sc->active = 0;
   sc->num_iters = 0;
   sc->parity = 0;
+  sc->nodal_done = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -289,7 +290,10 @@ __global__ void k_init_scalars(Batch B) {
 // ---------------------------------------------------------------------------
 __global__ void k_time_increment(Batch B) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (i < B.n) time_increment(B.sc + i);
+  if (i >= B.n) return;
+  DevScalars *sc = B.sc + i;
+  time_increment(sc);
+  if (i == 0 && B.dt_record) B.dt_record[sc->cycle - 1 - B.dt_base] = sc->deltatime;
 }
 
 // driver loop head (examples/benchmark.jl:82-84): the first iteration is
@@ -308,9 +312,10 @@ __global__ void k_run_begin(Batch B) {
 // ---------------------------------------------------------------------------
 // lagrangeNodal (src/domain.jl:1840-1872) as one kernel.
 //
-// A workgroup owns a column of NT x NT nodes and a chunk of cz node planes.
-// It marches up the chunk one element plane at a time: its 256 threads
-// compute the corner forces of the 16 x 16 elements around the node tile
+// A workgroup owns a column of (etx-1) x (ety-1) nodes (15 x 15 for large
+// domains) and a chunk of cz node planes.  It marches up the chunk one
+// element plane at a time: its threads compute the corner forces of the
+// etx x ety elements around the node tile
 // (integrateStressForElems + calcFBHourglassForceForElems arithmetic), stage
 // them in LDS, and the node threads fold them into per-node running sums.
 // A node's sum takes its elements in ascending element id -- the plane below
@@ -322,11 +327,9 @@ __global__ void k_run_begin(Batch B) {
 // position; src/domain.jl:1781-1838) into the other NodeSet, so tiles that
 // still read the old positions of shared halo nodes never see new values.
 // Elements on tile edges are computed by every tile that needs them
-// (16x16x(cz+1) per 15x15xcz nodes): no corner force ever goes to HBM.
+// (16x16x(cz+1) per 15x15xcz nodes for the 16 x 16 tile): no corner force
+// ever goes to HBM.
 // ---------------------------------------------------------------------------
-constexpr int ET = 16;  // element tile edge = threads per row
-constexpr int NT = NODAL_NT;
-
 __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
   // XCD-aware order: the dispatcher deals blocks round-robin over the 8 XCDs,
   // so give each XCD a contiguous run of tiles (their halos share its L2)
@@ -338,26 +341,28 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
   const InstDesc I = B.inst[inst_];
   DevScalars *sc = B.sc + inst_;
   if (B.gated && !sc->active) return;
-  const int s = I.s, tiles = I.tiles, quirks = B.quirks, keep_aux = B.keep_aux;
+  const int s = I.s, quirks = B.quirks, keep_aux = B.keep_aux;
+  const int etx = I.etx, ety = I.ety;
   const DevFields f = slice(B.f, I);
   const int lb = vb - I.gb0;
-  const int tx = lb % tiles, ty = (lb / tiles) % tiles, chunk = lb / (tiles * tiles);
-  const int i0 = tx * NT, j0 = ty * NT;
+  const int tx = lb % I.tx, ty = (lb / I.tx) % I.ty, chunk = lb / (I.tx * I.ty);
+  const int i0 = tx * (etx - 1), j0 = ty * (ety - 1);
   const int k0 = chunk * I.cz;
   const int k1 = min(k0 + I.cz, s + 1);
-  const int t = threadIdx.x, ti = t % ET, tj = t / ET;
+  const int t = threadIdx.x, ti = t % etx, tj = t / etx;
   const int s1 = s + 1;
-  // this thread's element in each plane, and its node column
+  // this thread's element in each plane, and its node column (threads past
+  // the etx x ety tile take part in the barriers only)
   const int ei = i0 - 1 + ti, ej = j0 - 1 + tj;
-  const bool elem_col = ei >= 0 && ei < s && ej >= 0 && ej < s;
+  const bool elem_col = tj < ety && ei >= 0 && ei < s && ej >= 0 && ej < s;
   const int ni = i0 + ti, nj = j0 + tj;
-  const bool node_col = ti < NT && tj < NT && ni <= s && nj <= s;
+  const bool node_col = ti < etx - 1 && tj < ety - 1 && ni <= s && nj <= s;
   // the four elements around the node column, in ascending id order:
   // (ni-1,nj-1) (ni,nj-1) (ni-1,nj) (ni,nj); LDS slot and node position
   // inside each (corner = ox + 2oy... mapped to the hex order below)
   const bool h00 = ni >= 1 && nj >= 1, h10 = ni < s && nj >= 1;
   const bool h01 = ni >= 1 && nj < s, h11 = ni < s && nj < s;
-  const int l00 = tj * ET + ti, l10 = l00 + 1, l01 = l00 + ET, l11 = l00 + ET + 1;
+  const int l00 = tj * etx + ti, l10 = l00 + 1, l01 = l00 + etx, l11 = l00 + etx + 1;
   // corner of the node in element (ox,oy) = node offset inside it:
   // (1,1)->2, (0,1)->3, (1,0)->1, (0,0)->0 at the bottom; +4 at the top
 
@@ -557,15 +562,19 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
     ch0 = nh0; ch1 = nh1; ch2 = nh2;
     ns0 = ns1 = ns2 = nh0 = nh1 = nh2 = 0.0;
   }
-}
-
-// the positions written by k_nodal become the current set
-__global__ void k_flip(Batch B) {
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (i >= B.n) return;
-  DevScalars *sc = B.sc + i;
-  if (B.gated && !sc->active) return;
-  sc->parity ^= 1;
+  // the instance's last workgroup to finish makes the node set just written
+  // current: every workgroup of the instance has then started (and read
+  // the old parity), and the next kernel sees the new one (a kernel
+  // boundary orders the stores; no fence is needed inside this launch)
+  if (B.flip) {
+    __syncthreads();
+    if (t == 0) {
+      if (atomicAdd(&sc->nodal_done, 1) == I.gbn - 1) {
+        sc->nodal_done = 0;
+        sc->parity = par ^ 1;
+      }
+    }
+  }
 }
 
 // calcKinematicsForElems (src/domain.jl:2099-2147), the deviatoric loop
@@ -1011,9 +1020,6 @@ hipError_t launch_kernel(int id, hipStream_t st, const Batch &b) {
     case KID_NODAL:
       // padded to a multiple of 8 for the XCD-aware block order
       hipLaunchKernelGGL(k_nodal, dim3(((b.gblocks + 7) / 8) * 8), dim3(BLOCK), 0, st, b);
-      break;
-    case KID_FLIP:
-      hipLaunchKernelGGL(k_flip, dim3(per_inst_grid(b.n)), dim3(64), 0, st, b);
       break;
     case KID_KIN:
       hipLaunchKernelGGL(k_kin, dim3(b.eblocks), dim3(BLOCK), 0, st, b);

@@ -35,7 +35,7 @@ SCALARS = ("time", "deltatime", "cycle", "dtcourant", "dthydro", "stoptime", "dt
            "v_cut", "qlc_monoq", "qqc_monoq", "qqc", "eosvmax", "eosvmin", "pmin", "emin",
            "dvovmax", "refdens")
 
-KERNELS = ("k_nodal", "k_kin", "k_qeos", "k_dt", "k_flip")
+KERNELS = ("k_nodal", "k_kin", "k_qeos", "k_dt")
 
 
 class LuleshError(Exception):

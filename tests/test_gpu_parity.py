@@ -94,6 +94,21 @@ def test_leapfrog_cycles_bitwise(nx, cycles):
     dom.close()
 
 
+@pytest.mark.parametrize("nx,cycles", [(16, 40), (48, 20), (63, 12), (96, 8)])
+def test_nodal_tile_shapes_bitwise(nx, cycles):
+    """Sizes whose k_nodal tile is not 16 x 16 elements (14 x 18 at s=16 and
+    48, 15 x 17 at 63, 12 x 21 at 96) and whose z-chunks are balanced: the
+    driver loop on the device ends bit-identical to the oracle."""
+    print(f"[tiles] nx={nx} cycles={cycles}", flush=True)
+    dom, od = make(nx, nthreads=8)
+    assert run(dom, cycles) == cycles
+    rc, _ = od.run(cycles)
+    assert rc == 0
+    assert_same(dom, od, STATE_E + STATE_N, f"tiles nx={nx}")
+    assert_scalars(dom, od, f"tiles nx={nx}")
+    dom.close()
+
+
 @pytest.mark.parametrize("nx", [8, 12])
 def test_run_to_stoptime(nx):
     """Driver loop to stoptime 1e-2 on the device vs the oracle's loop:
