@@ -137,10 +137,16 @@ def main():
         n = args.calibrate_copy * (1 << 20) // 8
         a = torch.ones(n, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
+        b.copy_(a)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
         for _ in range(5):
             b.copy_(a)
+        t1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"calibrate_copy_bytes": n * 8}))
+        ms = t0.elapsed_time(t1) / 5
+        print(json.dumps({"calibrate_copy_bytes": n * 8, "copy_ms": round(ms, 4),
+                          "copy_GB_s": round(2 * n * 8 / (ms * 1e-3) / 1e9, 1)}))
         return
     if args.gpus != 1:
         print("bench.py: this build runs one single-GPU domain; multi-GPU (--gpus > 1) is not "

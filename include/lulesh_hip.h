@@ -198,11 +198,14 @@ int lulesh_shadow_download(lulesh_ctx *ctx, const char *field, double *dst, int6
 #define LULESH_K_COUNT 4
 int lulesh_set_timing(lulesh_ctx *ctx, int32_t on);
 /* k_nodal geometry (no reference counterpart): each workgroup marches
- * `depth` node planes of its 15x15 node column.  0 restores the default
- * rule (deepest of 16/8/4/2 that still launches >= 1024 workgroups).
- * lulesh_autotune times k_nodal on the live state for depths 2..32 (the
- * kernel only writes the inactive node set, so the state is unchanged) and
- * keeps the fastest; *chosen and *best_ms (per launch) may be NULL. */
+ * `depth` node planes of its node column (15x15 nodes for large domains).
+ * 0 restores the default: per instance, node planes split into equal
+ * chunks whose depth minimises (launch rounds x planes per workgroup), a
+ * round being CUs x the workgroups one CU holds.  lulesh_autotune times
+ * k_nodal on the live state for the default and depths 2..32 (with its
+ * parity flip off the kernel only writes the inactive node set, so the
+ * state is unchanged) and keeps the fastest; *chosen (the depth of
+ * instance 0) and *best_ms (per launch) may be NULL. */
 int lulesh_set_nodal_depth(lulesh_ctx *ctx, int32_t depth);
 int32_t lulesh_nodal_depth(const lulesh_ctx *ctx);
 int lulesh_autotune(lulesh_ctx *ctx, int32_t *chosen, double *best_ms);

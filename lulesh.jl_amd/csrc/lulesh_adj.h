@@ -84,6 +84,24 @@ LHD int vd_slot(int call, bool q2) {
   return call;
 }
 
+// the volume derivative landing in slot `slot` (voluDer call `slot`,
+// except calls 1 and 3 swap slots without Q2)
+template <bool Q2>
+LHD void volume_derivative_slot(const double X[8], const double Y[8], const double Z[8], int slot,
+                                double &dx, double &dy, double &dz) {
+  const int c = (!Q2 && slot == 1) ? 3 : ((!Q2 && slot == 3) ? 1 : slot);
+  double x[6], y[6], z[6];
+  LUNROLL
+  for (int k = 0; k < 6; ++k) {
+    x[k] = X[vd_node(c, k)];
+    y[k] = Y[vd_node(c, k)];
+    z[k] = Z[vd_node(c, k)];
+  }
+  dx = vd_P(y, z) / 12.0;
+  dy = -vd_P(x, z) / 12.0;
+  dz = -vd_P(y, x) / 12.0;
+}
+
 template <bool Q2>
 LHD void volume_derivative_t(const double X[8], const double Y[8], const double Z[8],
                            double dvdx[8], double dvdy[8], double dvdz[8]) {
@@ -329,8 +347,6 @@ template <bool Q2, class Src, class Sink>
 LHD HgRev hourglass_rev_s(const double X[8], const double Y[8], const double Z[8], const Src &src,
                           Sink &sink, double determ, double ss, double mass, double hourg,
                           double cbrt_determ, double lX[8], double lY[8], double lZ[8]) {
-  double dvdx[8], dvdy[8], dvdz[8];
-  volume_derivative_t<Q2>(X, Y, Z, dvdx, dvdy, dvdz);
   const double volinv = 1.0 / determ;
   double hx[4], hy[4], hz[4];
   LUNROLL
@@ -353,9 +369,13 @@ LHD HgRev hourglass_rev_s(const double X[8], const double Y[8], const double Z[8
   for (int n = 0; n < 8; ++n) {
     const double v0 = src.v(0, n), v1 = src.v(1, n), v2 = src.v(2, n);
     const double f0 = src.lf(n, 0), f1 = src.lf(n, 1), f2 = src.lf(n, 2);
+    // the node's volume derivative is recomputed in each pass: holding all
+    // eight would keep 24 more values live through both passes
+    double dvx, dvy, dvz;
+    volume_derivative_slot<Q2>(X, Y, Z, n, dvx, dvy, dvz);
     LUNROLL
     for (int m = 0; m < 4; ++m) {
-      const double g = gamma_of(n, m) - volinv * (dvdx[n] * hx[m] + dvdy[n] * hy[m] + dvdz[n] * hz[m]);
+      const double g = gamma_of(n, m) - volinv * (dvx * hx[m] + dvy * hy[m] + dvz * hz[m]);
       H[0][m] += g * v0; H[1][m] += g * v1; H[2][m] += g * v2;
       A[m][0] += g * f0; A[m][1] += g * f1; A[m][2] += g * f2;
     }
@@ -374,9 +394,11 @@ LHD HgRev hourglass_rev_s(const double X[8], const double Y[8], const double Z[8
     const double v0 = src.v(0, n), v1 = src.v(1, n), v2 = src.v(2, n);
     const double f0 = src.lf(n, 0), f1 = src.lf(n, 1), f2 = src.lf(n, 2);
     double lv0 = 0.0, lv1 = 0.0, lv2 = 0.0, ldx = 0.0, ldy = 0.0, ldz = 0.0;
+    double dvx, dvy, dvz;
+    volume_derivative_slot<Q2>(X, Y, Z, n, dvx, dvy, dvz);
     LUNROLL
     for (int m = 0; m < 4; ++m) {
-      const double dh = dvdx[n] * hx[m] + dvdy[n] * hy[m] + dvdz[n] * hz[m];
+      const double dh = dvx * hx[m] + dvy * hy[m] + dvz * hz[m];
       const double g = gamma_of(n, m) - volinv * dh;
       lv0 += A[m][0] * g; lv1 += A[m][1] * g; lv2 += A[m][2] * g;
       const double lg = coef * (f0 * H[0][m] + f1 * H[1][m] + f2 * H[2][m] +
@@ -384,7 +406,7 @@ LHD HgRev hourglass_rev_s(const double X[8], const double Y[8], const double Z[8
       lvolinv -= lg * dh;
       const double s = volinv * lg;
       ldx -= s * hx[m]; ldy -= s * hy[m]; ldz -= s * hz[m];
-      lhx[m] -= s * dvdx[n]; lhy[m] -= s * dvdy[n]; lhz[m] -= s * dvdz[n];
+      lhx[m] -= s * dvx; lhy[m] -= s * dvy; lhz[m] -= s * dvz;
     }
     sink.lv(0, n, coef * lv0);
     sink.lv(1, n, coef * lv1);

@@ -46,6 +46,7 @@ struct lulesh_ctx {
   Batch b{};                      // device pointers + launch geometry
   DevScalars *hsc = nullptr;      // pinned host mirror [n]
   bool partials_valid = false;
+  int cus = 256, nodal_per_cu = 2;  // compute units; k_nodal workgroups resident per CU
   int keep_aux = 0;
   // timing
   bool timing = false;
@@ -273,20 +274,41 @@ int balanced_depth(int s, int depth) {
   return (m + chunks - 1) / chunks;
 }
 
-int plan_nodal(lulesh_ctx *c, int cz) {
-  // default: the deepest of 16/8/4/2 that still launches >= 1024 workgroups,
-  // balanced per instance; an explicit depth applies unchanged to all
-  const bool automatic = cz <= 0;
-  if (automatic) {
-    cz = 16;
-    for (;;) {
-      int64_t total = 0;
-      for (const InstDesc &I : c->inst)
-        total += nodal_blocks(NodalTile{I.etx, I.ety, I.tx, I.ty}, I.s, balanced_depth(I.s, cz));
-      if (total >= 1024 || cz <= 2) break;
-      cz /= 2;
+// Default z-march depth.  A k_nodal workgroup's time grows with the planes
+// it marches (depth + 1 element planes), and the launch runs in rounds of
+// (CUs x resident workgroups per CU): the estimate
+//   ceil(workgroups / slots) * (depth + 1)
+// is minimised over all depths (the smaller depth on ties, for balance).
+// At s=192 on 256 CUs x 2 this picks 3 chunks of 65 planes (507 workgroups,
+// one round) over 13 chunks of 15 (2197 workgroups, 4.3 rounds).
+int auto_depth(const lulesh_ctx *c) {
+  int maxm = 2;
+  for (const InstDesc &I : c->inst) maxm = I.s + 1 > maxm ? I.s + 1 : maxm;
+  const int64_t slots = (int64_t)c->cus * c->nodal_per_cu;
+  int best = 2;
+  int64_t best_cost = -1;
+  for (int d = 2; d <= maxm; ++d) {
+    int64_t total = 0;
+    int deepest = 0;
+    for (const InstDesc &I : c->inst) {
+      const int cz = balanced_depth(I.s, d);
+      total += nodal_blocks(NodalTile{I.etx, I.ety, I.tx, I.ty}, I.s, cz);
+      deepest = cz > deepest ? cz : deepest;
+    }
+    const int64_t cost = ((total + slots - 1) / slots) * (deepest + 1);
+    if (best_cost < 0 || cost < best_cost) {
+      best = d;
+      best_cost = cost;
     }
   }
+  return best;
+}
+
+int plan_nodal(lulesh_ctx *c, int cz) {
+  // default: auto_depth, balanced per instance; an explicit depth applies
+  // unchanged to all instances
+  const bool automatic = cz <= 0;
+  if (automatic) cz = auto_depth(c);
   std::vector<int> gblock_inst;
   int gb = 0;
   for (int i = 0; i < c->n; ++i) {
@@ -333,6 +355,15 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
   c->n = n;
   c->quirks = quirks;
   c->device = device;
+  {
+    int cus = 0;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus < 1)
+      cus = 256;
+    c->cus = cus;
+    c->nodal_per_cu = nodal_blocks_per_cu();
+  }
   c->inst.resize(n);
   int eb = 0, nb = 0, gb_max = 0;
   int64_t e0 = 0, n0 = 0;
@@ -1195,7 +1226,7 @@ int lulesh_autotune(lulesh_ctx *c, int32_t *chosen, double *best_ms) {
   // k_nodal with its parity flip switched off reads the current node set
   // and writes only the other one, so it can be timed on the live state
   // without changing it
-  const int cand[] = {2, 4, 8, 16, 32};
+  const int cand[] = {0 /* the model's choice */, 2, 4, 8, 16, 32};
   int best = -1;
   float best_t = 0.f;
   hipEvent_t ea = get_event(c), eb = get_event(c);
@@ -1233,7 +1264,7 @@ int lulesh_autotune(lulesh_ctx *c, int32_t *chosen, double *best_ms) {
   if (!rc) rc = clear_err(c);
   if (!rc) rc = finish(c, false);
   if (rc) return rc;
-  if (chosen) *chosen = best;
+  if (chosen) *chosen = c->inst[0].cz;
   if (best_ms) *best_ms = best_t / 5.0;
   return 0;
 }

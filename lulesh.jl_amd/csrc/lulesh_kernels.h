@@ -123,6 +123,8 @@ inline int nodal_blocks(const NodalTile &t, int s, int cz) {
 }
 
 hipError_t launch_init(hipStream_t st, const Batch &b);
+// workgroups of k_nodal one CU holds at once (register and LDS limits)
+int nodal_blocks_per_cu();
 hipError_t launch_kernel(int id, hipStream_t st, const Batch &b);
 hipError_t launch_math(hipStream_t st, int op, const double *a, const double *b, double *out,
                        int64_t n);

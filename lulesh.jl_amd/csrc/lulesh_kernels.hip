@@ -1001,6 +1001,13 @@ namespace lulesh {
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 static inline unsigned per_inst_grid(int n) { return (unsigned)((n + 63) / 64); }
 
+int nodal_blocks_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_nodal, BLOCK, 0) != hipSuccess || n < 1)
+    n = 1;
+  return n;
+}
+
 hipError_t launch_init(hipStream_t st, const Batch &b) {
   hipLaunchKernelGGL(k_init_nodes, dim3(b.nblocks), dim3(BLOCK), 0, st, b);
   hipLaunchKernelGGL(k_init_elems, dim3(b.eblocks), dim3(BLOCK), 0, st, b);

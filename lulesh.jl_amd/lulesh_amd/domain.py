@@ -136,6 +136,21 @@ class Domain:
     def reset(self):
         _check(self._lib, self._ctx, self._lib.lulesh_reset(self._ctx))
 
+    # -- k_nodal geometry (no reference counterpart; whole context) --
+    def autotune(self):
+        """Time k_nodal's z-march depths on the live state and keep the
+        fastest; returns (depth, ms per launch)."""
+        ch, ms = ctypes.c_int32(), ctypes.c_double()
+        _check(self._lib, self._ctx,
+               self._lib.lulesh_autotune(self._ctx, ctypes.byref(ch), ctypes.byref(ms)))
+        return ch.value, ms.value
+
+    def set_nodal_depth(self, depth):
+        _check(self._lib, self._ctx, self._lib.lulesh_set_nodal_depth(self._ctx, int(depth)))
+
+    def nodal_depth(self):
+        return int(self._lib.lulesh_nodal_depth(self._ctx))
+
     # -- field access by reference name --
     def _len(self, name):
         if name in ELEM_FIELDS:
