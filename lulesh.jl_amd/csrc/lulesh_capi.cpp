@@ -287,7 +287,7 @@ int auto_depth(const lulesh_ctx *c) {
   const int64_t slots = (int64_t)c->cus * c->nodal_per_cu;
   int best = 2;
   int64_t best_cost = -1;
-  for (int d = 2; d <= maxm; ++d) {
+  for (int d = 1; d <= maxm; ++d) {
     int64_t total = 0;
     int deepest = 0;
     for (const InstDesc &I : c->inst) {

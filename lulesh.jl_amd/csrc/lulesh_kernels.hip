@@ -331,11 +331,13 @@ __global__ void k_run_begin(Batch B) {
 // ever goes to HBM.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
-  // XCD-aware order: the dispatcher deals blocks round-robin over the 8 XCDs,
-  // so give each XCD a contiguous run of tiles (their halos share its L2)
+  // XCD-aware order for one domain: the dispatcher deals blocks round-robin
+  // over the 8 XCDs, so give each XCD a contiguous run of tiles (their halos
+  // share its L2; -8 % at s=192).  A batch keeps the plain order, which
+  // spreads every instance's workgroups -- of unequal cost -- over all XCDs.
   const int nblocks = B.gblocks;
   const int per = (nblocks + 7) / 8;
-  const int vb = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  const int vb = B.n == 1 ? (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8) : (int)blockIdx.x;
   if (vb >= nblocks) return;
   const int inst_ = B.gblock_inst[vb];
   const InstDesc I = B.inst[inst_];
