@@ -101,7 +101,7 @@ def test_autotune_leaves_state_unchanged_and_results_identical():
     b.run(30)
     before = {(i, f): b.instance(i).get(f) for i in range(len(NX)) for f in FIELDS}
     depth, ms = b.autotune()
-    assert depth in (2, 4, 8, 16, 32) and ms > 0
+    assert depth >= 1 and ms > 0  # instance 0's depth: the model's or a fixed one
     for (i, f), a in before.items():
         assert np.array_equal(bits(a), bits(b.instance(i).get(f))), (i, f)
     b.run(60)
@@ -112,3 +112,30 @@ def test_autotune_leaves_state_unchanged_and_results_identical():
         same(b.instance(i), ref.instance(i), f"depth-independence {i}")
     b.close()
     ref.close()
+
+
+def test_error_stops_only_its_instance():
+    """An invalid elemBC mask in instance 1 raises the reference's "Error"
+    there (code 6) in the first cycle; the other instances run on and end
+    exactly as lone runs."""
+    print("[batch] per-instance error", flush=True)
+    b = Batch([8, 10, 12])
+    bc = b.instance(1).get("elemBC")
+    bc[5] |= 0x1 | 0x2  # XI_M_SYMM | XI_M_FREE: no valid case
+    b.instance(1).set("elemBC", bc)
+    cyc = b.run(50, check=False)
+    assert b.errors() == [0, 6, 0]
+    assert cyc[0] == 50 and cyc[2] == 50 and cyc[1] == 1
+    for i, nx in ((0, 8), (2, 12)):
+        lone = Batch([nx])
+        lone.run(50)
+        same(b.instance(i), lone.instance(0), f"instance {i} next to a failing one")
+        lone.close()
+    with pytest.raises(Exception) as ei:
+        b2 = Batch([8, 10])
+        bc = b2.instance(0).get("elemBC")
+        bc[5] |= 0x1 | 0x2
+        b2.instance(0).set("elemBC", bc)
+        b2.run(5)
+    assert "Error" in str(ei.value)
+    b.close()
