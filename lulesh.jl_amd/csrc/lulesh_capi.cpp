@@ -549,6 +549,21 @@ int adj_alloc(lulesh_ctx *c, double **p, int64_t n) {
   return 0;
 }
 
+// replaces a growable adjoint buffer (the old one is freed)
+int adj_regrow(lulesh_ctx *c, double **p, int64_t n) {
+  if (*p) {
+    auto &m = c->adj.mem;
+    for (size_t i = 0; i < m.size(); ++i)
+      if (m[i] == *p) {
+        (void)hipFree(m[i]);
+        m.erase(m.begin() + (long)i);
+        break;
+      }
+    *p = nullptr;
+  }
+  return adj_alloc(c, p, n);
+}
+
 // one cycle of a segment: its start state S_k (6N + 5E, the checkpoint
 // layout) and its temporaries (9E); a segment of L cycles uses L + 1 slots,
 // the last holding only the state after it
@@ -565,7 +580,7 @@ int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
   const int64_t state = 6 * N + 5 * E;
   int rc = 0;
   if (ncycles > B.dt_cap) {
-    rc = adj_alloc(c, &B.dt, ncycles);
+    rc = adj_regrow(c, &B.dt, ncycles);
     if (rc) return rc;
     B.dt_cap = ncycles;
   }
@@ -609,12 +624,12 @@ int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
   }
   const int64_t nck = (ncycles + every - 1) / every;
   if (nck > B.ckpt_cap) {
-    rc = adj_alloc(c, &B.ckpt, nck * state);
+    rc = adj_regrow(c, &B.ckpt, nck * state);
     if (rc) return rc;
     B.ckpt_cap = nck;
   }
   if (every > B.seg_cap) {
-    rc = adj_alloc(c, &B.seg, (every + 1) * cycle_slot(c));
+    rc = adj_regrow(c, &B.seg, (every + 1) * cycle_slot(c));
     if (rc) return rc;
     B.seg_cap = every;
   }
