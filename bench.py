@@ -200,7 +200,7 @@ def main():
     adjoint = None if args.no_adjoint else bench_adjoint()
     batched = None if args.no_batched else bench_batched()
 
-    cpu = None
+    cpu = cpu_par = None
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         v, secs = cpu_baseline(nx, args.cpu_cycles, args.cpu_threads)
@@ -208,6 +208,11 @@ def main():
                "kind": "port",
                "sample": f"CPU oracle (C++ restatement), s={nx}, cycles 1-{args.cpu_cycles} of the "
                          f"same workload, {secs:.1f} s, {args.cpu_threads} thread(s)"}
+        # the same sample over the host's cores (OpenMP; bit-identical results)
+        nthr = max(1, min(16, os.cpu_count() or 1))
+        vp, secs_p = cpu_baseline(nx, args.cpu_cycles, nthr)
+        cpu_par = {"value": round(vp, 1), "unit": "zone-cycles/s", "cores": nthr,
+                   "sample": f"same sample, {secs_p:.1f} s, {nthr} OpenMP threads"}
 
     out = {
         "metric": "FP64 zone-cycles/s at s=192; adjoint/forward time ratio; batched speedup",
@@ -226,6 +231,10 @@ def main():
                    "nodes": N, "cycles": ncyc, "quirks": "reference"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_parallel": cpu_par,
+        # verifyAndWriteFinalOutput's timing lines (src/utils.jl:128-129,161-162)
+        "reference_report": {"grind_time_us_per_zone_cycle": round(wall * 1e6 / ncyc / E, 8),
+                             "fom_z_per_s": round(1000.0 / (wall * 1e6 / ncyc / E), 3)},
         "kernels": kernels,
         "adjoint": adjoint,
         "batched": batched,

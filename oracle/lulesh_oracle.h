@@ -6,6 +6,22 @@
  * check and the CPU-baseline leg of bench.py may load it, and only as the
  * checker: it is never the thing measured or shipped.
  *
+ * How each part is pinned (tests/test_oracle_pins.py; none re-types a
+ * formula of this file): cbrt -- exact integer arithmetic; element volume
+ * -- unit cube, parallelepipeds, Gauss quadrature; volume derivative --
+ * finite differences (slots 2/4 pin quirk Q2); node normals -- zero sum /
+ * gradient of the volume, face-5 closed form (quirk Q1); shape derivatives,
+ * velocity gradient, characteristic length -- affine/linear closed forms;
+ * FB hourglass -- affine motion and pure modes on a cube; MonoQ gradients --
+ * box and parallelepiped closed forms; MonoQ limiter -- closed forms of
+ * every branch; EOS -- rest state and the compression energy balance;
+ * timeIncrement! -- branch table; Domain(prob) -- independent
+ * constructions; cycle 1 -- closed form; the whole quirk-free cycle -- the
+ * LULESH 2.0 published s=30 result (932 iterations, e = 2.025075e+05);
+ * quirk Q3 -- changes results and keeps the reference's non-termination at
+ * s=30.  Parity unpinned at the last ulp: Julia's cbrt (not correctly
+ * rounded) and its @simd reassociation of the <= 8-term corner gathers.
+ *
  * Index conventions: every array here is 0-based in memory.  Index *values*
  * keep the reference's stored bases: nodelist, the six face-neighbour lists,
  * the symm sets and nodeElemStart hold 0-based ids, nodeElemCornerList and matElemlist hold

@@ -52,6 +52,8 @@ def load():
                                                  ctypes.c_int, _PD]),
         "oracle_eval_eos_elem": (None, [_D] * 8 + [_PD]),
         "oracle_time_increment_scalars": (None, [_PD]),
+        "oracle_monoq_gradients_elem": (None, [_PD] * 6 + [_D, _D, _PD]),
+        "oracle_monoq_region_elem": (None, [_PD] * 4 + [_D] * 6 + [_PD]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -227,3 +229,21 @@ def time_increment_scalars(time, deltatime, cycle, dtcourant, dthydro, stoptime=
                   dtype=np.float64)
     load().oracle_time_increment_scalars(_p(st))
     return st[0], st[1], int(st[2])
+
+
+def monoq_gradients(x, y, z, xv, yv, zv, volo, vnew):
+    """{delx_xi, delx_eta, delx_zeta, delv_xi, delv_eta, delv_zeta} of one element."""
+    x, y, z, xv, yv, zv = (arr(a) for a in (x, y, z, xv, yv, zv))
+    out = np.zeros(6)
+    load().oracle_monoq_gradients_elem(_p(x), _p(y), _p(z), _p(xv), _p(yv), _p(zv), float(volo),
+                                       float(vnew), _p(out))
+    return out
+
+
+def monoq_region(dv, dm, dp, dx, vdov, elem_mass, volo, vnew, qlc=0.5, qqc=2.0 / 3.0):
+    """(qlin, qquad) of one element; dv/dm/dp/dx per direction xi, eta, zeta."""
+    dv, dm, dp, dx = (arr(a) for a in (dv, dm, dp, dx))
+    q = np.zeros(2)
+    load().oracle_monoq_region_elem(_p(dv), _p(dm), _p(dp), _p(dx), float(vdov), float(elem_mass),
+                                    float(volo), float(vnew), float(qlc), float(qqc), _p(q))
+    return q[0], q[1]

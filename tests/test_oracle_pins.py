@@ -237,6 +237,71 @@ def test_hourglass_quirk_q2_acts_only_off_parallelepipeds():
 
 # ---------------- EOS ----------------
 
+def test_monoq_gradients_box_closed_form():
+    """Axis-aligned box with edges (hx, hy, hz) and velocity (a x, b y, c z):
+    the characteristic lengths are the edges and the directional velocity
+    gradients are a, b, c (src/domain.jl:2175-2320)."""
+    hx, hy, hz = 0.5, 0.25, 2.0
+    a, b, c = -3.0, 1.5, 0.75
+    P = UNIT * np.array([hx, hy, hz]) + np.array([1.0, -2.0, 0.5])
+    V = (P - P[0]) * np.array([a, b, c])
+    vol = hx * hy * hz
+    out = O.monoq_gradients(P[:, 0], P[:, 1], P[:, 2], V[:, 0], V[:, 1], V[:, 2], vol, 1.0)
+    assert out == pytest.approx([hx, hy, hz, a, b, c], rel=1e-14, abs=1e-14)
+
+
+def test_monoq_gradients_parallelepiped_heights():
+    """For x = A u over the unit cube, delx_d is the distance between the two
+    faces normal to logical direction d: |det A| / |A_j x A_k|; a uniform
+    dilation v = g x has every directional gradient delv_d = g."""
+    rng = np.random.default_rng(5)
+    A = np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+    if np.linalg.det(A) < 0:
+        A[:, 0] *= -1
+    P = UNIT @ A.T
+    vol = abs(np.linalg.det(A))
+    g = -0.4
+    V = g * P
+    out = O.monoq_gradients(P[:, 0], P[:, 1], P[:, 2], V[:, 0], V[:, 1], V[:, 2], vol, 1.0)
+    cols = [A[:, 0], A[:, 1], A[:, 2]]
+    heights = [vol / np.linalg.norm(np.cross(cols[(d + 1) % 3], cols[(d + 2) % 3])) for d in range(3)]
+    assert out[:3] == pytest.approx(heights, rel=1e-12)
+    assert out[3:] == pytest.approx([g, g, g], rel=1e-12)
+
+
+def test_monoq_limiter_closed_forms():
+    """calcMonotonicQRegionForElems (src/domain.jl:2322-2513): phi =
+    clamp(min((dm+dp)/2, 2dm, 2dp) / dv, 0, 1); q_lin = -qlc rho sum dv dx
+    (1-phi), q_quad = qqc rho sum (dv dx)^2 (1-phi^2), both 0 when the
+    element expands."""
+    dv = np.array([-0.3, -0.2, -0.5])
+    dx = np.array([0.1, 0.2, 0.05])
+    mass, volo, vnew = 2.0, 0.5, 0.8
+    rho = mass / (volo * vnew)
+    qlc, qqc = 0.5, 2.0 / 3.0
+    lin = (dv * dx).sum()
+    quad = ((dv * dx) ** 2).sum()
+    # expansion: no q
+    assert O.monoq_region(dv, dv, dv, dx, 0.1, mass, volo, vnew) == (0.0, 0.0)
+    # smooth (linear) field, or steeper neighbours: phi = 1, no q
+    for k in (1.0, 3.0):
+        ql, qq = O.monoq_region(dv, k * dv, k * dv, dx, -1.0, mass, volo, vnew)
+        assert ql == 0.0 and qq == 0.0
+    # isolated jump, or neighbours of the opposite sign: phi = 0
+    for dm, dp in ((0 * dv, 0 * dv), (-dv, dv), (dv, 0 * dv)):
+        ql, qq = O.monoq_region(dv, dm, dp, dx, -1.0, mass, volo, vnew)
+        assert ql == pytest.approx(-qlc * rho * lin, rel=1e-14)
+        assert qq == pytest.approx(qqc * rho * quad, rel=1e-14)
+    # neighbours at a quarter of the element's gradient: phi = 1/4
+    ql, qq = O.monoq_region(dv, 0.25 * dv, 0.25 * dv, dx, -1.0, mass, volo, vnew)
+    assert ql == pytest.approx(-qlc * rho * lin * 0.75, rel=1e-14)
+    assert qq == pytest.approx(qqc * rho * quad * (1 - 1 / 16), rel=1e-14)
+    # unequal neighbours: the limiter multiplier 2 decides, phi = 2 * 0.3
+    ql, qq = O.monoq_region(dv, 0.3 * dv, 1.5 * dv, dx, -1.0, mass, volo, vnew)
+    assert ql == pytest.approx(-qlc * rho * lin * 0.4, rel=1e-13)
+    assert qq == pytest.approx(qqc * rho * quad * (1 - 0.36), rel=1e-13)
+
+
 def test_eos_at_rest_closed_form():
     for e0 in (1.0, 1.169998815e7, 3.0e9):
         p, e, q, ss = O.eos_elem(1.0, 0.0, e0, 0.0, 0.0, 0.0, 0.0)
