@@ -307,3 +307,35 @@ def test_device_sqrt_and_division_are_ieee():
     assert np.array_equal(bits(out), bits(np.sqrt(a)))
     assert L.lulesh_math_check(2, a.ctypes.data_as(P), b.ctypes.data_as(P), out.ctypes.data_as(P), a.size, 0) == 0
     assert np.array_equal(bits(out), bits(a / b))
+
+
+# ---- seeded random configurations ----
+
+def _random_cases(n, seed):
+    """(nx, quirks, cycles) drawn from numpy's default_rng(seed) in that
+    order: nx ~ U{1..24}, quirks ~ U{0..7}, cycles ~ U{1..60}."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.integers(1, 25)), int(rng.integers(0, 8)), int(rng.integers(1, 61)))
+            for _ in range(n)]
+
+
+@pytest.mark.parametrize("nx,quirks,cycles", _random_cases(12, 2026))
+def test_random_configurations_bitwise(nx, quirks, cycles):
+    """Every quirk combination, sizes from 1 to 24 (all k_nodal tile shapes
+    and depths those sizes get) and cycle counts from 1 to 60: the driver
+    loop ends bit-identical to the oracle's, or both raise the same error."""
+    print(f"[random] nx={nx} quirks={quirks} cycles={cycles}", flush=True)
+    dom, od = make(nx, quirks=quirks, nthreads=4)
+    try:
+        ran = run(dom, cycles)
+        err = 0
+    except LuleshError as exc:
+        ran, err = None, exc.code
+    rc, _ = od.run(cycles)
+    assert err == rc
+    if not err:
+        assert ran == od.get("cycle")
+        assert_same(dom, od, STATE_E + STATE_N, f"random nx={nx} q={quirks}")
+        assert_scalars(dom, od, f"random nx={nx} q={quirks}")
+    dom.close()
+    od.close()
