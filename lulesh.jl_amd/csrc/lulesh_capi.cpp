@@ -63,6 +63,11 @@ struct lulesh_ctx {
   int graph_keep_aux = -1;
   std::string last_error;
   std::vector<void *> allocations;
+  std::vector<int> gblock_host;  // instance of each k_nodal block
+  int *gorder = nullptr;         // k_nodal dispatch table (regrown by plan_nodal)
+  int64_t gorder_cap = 0;
+  std::vector<int> gorder_host;
+  std::vector<char> order_live;  // instances the table holds (empty: all)
   AdjBuffers adj;
 };
 
@@ -230,6 +235,7 @@ void release(lulesh_ctx *c) {
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   for (void *p : c->allocations) (void)hipFree(p);
   for (void *p : c->adj.mem) (void)hipFree(p);
+  if (c->gorder) (void)hipFree(c->gorder);
   if (c->hsc) (void)hipHostFree(c->hsc);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -304,6 +310,63 @@ int auto_depth(const lulesh_ctx *c) {
   return best;
 }
 
+// k_nodal dispatch order.  The dispatcher deals a launch's workgroups
+// round-robin over the eight XCDs (workgroup g to XCD g % 8), each with its
+// own L2.  XCD x gets the x-th of eight contiguous runs of the block order
+// (instance, chunk, tile row, tile), cut at equal sums of marched element
+// planes: neighbouring tiles, which read each other's halo nodes, share an
+// L2, and every XCD gets the same work.  `live` (null: all) restricts the
+// table to the instances still running, so finished ones leave no hole in
+// one XCD's share.  The first call for a plan (glaunch 0) sizes the launch;
+// a later one that would need more workgroups than it has lists the live
+// blocks in plain order instead.  Synchronous.
+int order_nodal(lulesh_ctx *c, const std::vector<char> *live) {
+  const std::vector<int> &gi = c->gblock_host;
+  std::vector<int> lv;
+  lv.reserve(gi.size());
+  int64_t total = 0;
+  for (int v = 0; v < (int)gi.size(); ++v)
+    if (!live || (*live)[gi[v]]) {
+      lv.push_back(v);
+      total += c->inst[gi[v]].cz + 1;
+    }
+  const int nl = (int)lv.size();
+  int cut[9];
+  cut[0] = 0;
+  int x = 1;
+  int64_t acc = 0;
+  for (int k = 0; k < nl; ++k) {
+    acc += c->inst[gi[lv[k]]].cz + 1;
+    while (x < 8 && acc * 8 >= total * x) cut[x++] = k + 1;
+  }
+  while (x <= 8) cut[x++] = nl;
+  int runlen = 0;
+  for (int r = 0; r < 8; ++r) runlen = cut[r + 1] - cut[r] > runlen ? cut[r + 1] - cut[r] : runlen;
+  if (c->b.glaunch == 0) c->b.glaunch = runlen * 8 > 8 ? runlen * 8 : 8;
+  std::vector<int> &order = c->gorder_host;
+  order.assign((size_t)c->b.glaunch, -1);
+  if (runlen * 8 <= c->b.glaunch) {
+    for (int r = 0; r < 8; ++r)
+      for (int k = 0; k < cut[r + 1] - cut[r]; ++k) order[(size_t)k * 8 + r] = lv[cut[r] + k];
+  } else {
+    for (int k = 0; k < nl; ++k) order[k] = lv[k];
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if ((int64_t)order.size() > c->gorder_cap) {
+    if (c->gorder) HIP_TRY(c, hipFree(c->gorder));
+    c->gorder = nullptr;
+    c->gorder_cap = 0;
+    HIP_TRY(c, hipMalloc((void **)&c->gorder, order.size() * sizeof(int)));
+    c->gorder_cap = (int64_t)order.size();
+  }
+  c->b.gblock_order = c->gorder;
+  HIP_TRY(c, hipMemcpyAsync(c->gorder, order.data(), order.size() * sizeof(int),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->order_live = live ? *live : std::vector<char>();
+  return 0;
+}
+
 int plan_nodal(lulesh_ctx *c, int cz) {
   // default: auto_depth, balanced per instance; an explicit depth applies
   // unchanged to all instances
@@ -320,11 +383,14 @@ int plan_nodal(lulesh_ctx *c, int cz) {
     gblock_inst.insert(gblock_inst.end(), I.gbn, i);
   }
   c->b.gblocks = gb;
+  c->gblock_host = std::move(gblock_inst);
+  c->b.glaunch = 0;
   HIP_TRY(c, hipMemcpyAsync((void *)c->b.inst, c->inst.data(), sizeof(InstDesc) * c->n,
                             hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(c, hipMemcpyAsync((void *)c->b.gblock_inst, gblock_inst.data(), sizeof(int) * gb,
+  HIP_TRY(c, hipMemcpyAsync((void *)c->b.gblock_inst, c->gblock_host.data(), sizeof(int) * gb,
                             hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const int rc = order_nodal(c, nullptr);
+  if (rc) return rc;
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -470,7 +536,7 @@ int get_inst_scalars(lulesh_ctx *c, int32_t inst, DevScalars *out) {
   return 0;
 }
 
-int run_cycles(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run /* [n] or null */) {
+int run_cycles_body(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
   HIP_TRY(c, hipSetDevice(c->device));
   int rc = pull_scalars(c);
   if (!rc) rc = finish(c, false);
@@ -523,6 +589,17 @@ int run_cycles(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run /* [n] or n
     if (!any) break;
     next_cycle = mn;
     if (cap < 256) cap *= 2;
+    // instances that finished drop out of k_nodal's dispatch table
+    std::vector<char> live(c->n);
+    bool all = true;
+    for (int i = 0; i < c->n; ++i) {
+      live[i] = c->hsc[i].active != 0;
+      all = all && live[i];
+    }
+    if (all ? !c->order_live.empty() : live != c->order_live) {
+      rc = order_nodal(c, all ? nullptr : &live);
+      if (rc) return rc;
+    }
   }
   c->partials_valid = false;
   int err = ERR_NONE;
@@ -535,6 +612,17 @@ int run_cycles(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run /* [n] or n
     return fail(c, LULESH_ERR_HIP, "unknown device error code");
   }
   return 0;
+}
+
+// The driver loop; k_nodal's dispatch table is back to every instance on
+// return, as the other entry points launch it for all of them.
+int run_cycles(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run /* [n] or null */) {
+  int rc = run_cycles_body(c, num_iters, cycles_run);
+  if (!c->order_live.empty()) {
+    const int r2 = order_nodal(c, nullptr);
+    if (!rc) rc = r2;
+  }
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

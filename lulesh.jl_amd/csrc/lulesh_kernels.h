@@ -51,7 +51,7 @@ struct DevFields {
 // one instance of a (ragged) batch
 struct InstDesc {
   int s;                 // elements per edge
-  int etx, ety;          // k_nodal tile: etx x ety elements around (etx-1) x (ety-1) nodes
+  int etx, ety;          // k_nodal tile: etx x ety element lanes (nodal_span)
   int tx, ty;            // k_nodal tiles along x and y
   int64_t e0, n0;        // offsets into the flat arrays
   int64_t E, N;          // sizes
@@ -69,7 +69,9 @@ struct Batch {
   const int *eblock_inst;   // instance of each element-kernel block
   const int *nblock_inst;   // instance of each node-kernel block
   const int *gblock_inst;   // instance of each k_nodal block
+  const int *gblock_order;  // k_nodal block run by each launched workgroup, -1: none
   int eblocks, nblocks, gblocks;
+  int glaunch;              // k_nodal workgroups launched (a multiple of 8)
   DevScalars *sc;           // [n]
   DevFields f;
   double *dtpart;           // 2 per element-kernel block
@@ -95,26 +97,36 @@ enum KernelId {
 };
 
 constexpr int BLOCK = 256;
-// k_nodal tiling: a workgroup owns an (etx-1) x (ety-1) column of nodes and
-// computes the etx x ety elements around it (etx * ety <= BLOCK), marching
-// `cz` node planes up the column.  Per instance the tile is the one that
-// covers the (s+1)^2 node columns with the fewest tiles (16 x 16 elements
-// for large s; the squarest among equals), so small domains do not leave
-// most of a 16 x 16 tile idle.
+// k_nodal tiling: a workgroup marches a column of etx x ety element lanes
+// (etx * ety <= BLOCK) up `cz` node planes and owns the node columns whose
+// elements all lie in its column.  Along an axis of s elements (s+1 nodes),
+// with a lanes per tile: the first tile's elements start at node 0 and it
+// owns a nodes (no element precedes node 0); an inner tile's first element
+// is the halo shared with the tile before and it owns a-1 nodes; the last
+// tile owns the rest, at most a (no element follows node s); one tile covers
+// the axis when s+1 <= a.
+inline int nodal_span(int s, int a) {
+  const int m = s + 1;
+  if (m <= a) return 1;
+  const int rest = m - 2 * a;
+  return 2 + (rest > 0 ? (rest + a - 2) / (a - 1) : 0);
+}
 struct NodalTile {
   int etx, ety, tx, ty;
 };
+// Per instance: the tile, etx in 16..32 lanes wide (whole cache-line runs
+// of a row), ety = BLOCK / etx, that covers the (s+1)^2 node columns with
+// the fewest tiles (the squarest among equals).  At s=192 that is 17 x 15
+// (12 x 14 tiles); small domains avoid a mostly idle last tile.
 inline NodalTile nodal_tile(int s) {
-  const int m = s + 1;
-  NodalTile best{16, 16, (m + 14) / 15, (m + 14) / 15};
-  for (int a = 1; a + 1 <= BLOCK; ++a) {
-    const int b = BLOCK / (a + 1) - 1;
-    if (b < 1) break;
-    const int tx = (m + a - 1) / a, ty = (m + b - 1) / b;
-    const int cur = best.tx * best.ty, cand = tx * ty;
-    const int dcur = best.etx > best.ety ? best.etx - best.ety : best.ety - best.etx;
+  NodalTile best{0, 0, 0, 0};
+  for (int a = 16; a <= 32; ++a) {
+    const int b = BLOCK / a;
+    const int tx = nodal_span(s, a), ty = nodal_span(s, b);
     const int dcand = a > b ? a - b : b - a;
-    if (cand < cur || (cand == cur && dcand < dcur)) best = NodalTile{a + 1, b + 1, tx, ty};
+    const int dcur = best.etx > best.ety ? best.etx - best.ety : best.ety - best.etx;
+    if (best.etx == 0 || tx * ty < best.tx * best.ty || (tx * ty == best.tx * best.ty && dcand < dcur))
+      best = NodalTile{a, b, tx, ty};
   }
   return best;
 }

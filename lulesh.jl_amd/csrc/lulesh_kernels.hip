@@ -312,10 +312,10 @@ __global__ void k_run_begin(Batch B) {
 // ---------------------------------------------------------------------------
 // lagrangeNodal (src/domain.jl:1840-1872) as one kernel.
 //
-// A workgroup owns a column of (etx-1) x (ety-1) nodes (15 x 15 for large
-// domains) and a chunk of cz node planes.  It marches up the chunk one
-// element plane at a time: its threads compute the corner forces of the
-// etx x ety elements around the node tile
+// A workgroup owns a column of nodes (up to etx x ety; 16 x 14 for inner
+// tiles at s=192, see nodal_tile) and a chunk of cz node planes.  It marches
+// up the chunk one element plane at a time: its threads compute the corner
+// forces of the etx x ety elements around the node tile
 // (integrateStressForElems + calcFBHourglassForceForElems arithmetic), stage
 // them in LDS, and the node threads fold them into per-node running sums.
 // A node's sum takes its elements in ascending element id -- the plane below
@@ -327,18 +327,15 @@ __global__ void k_run_begin(Batch B) {
 // position; src/domain.jl:1781-1838) into the other NodeSet, so tiles that
 // still read the old positions of shared halo nodes never see new values.
 // Elements on tile edges are computed by every tile that needs them
-// (16x16x(cz+1) per 15x15xcz nodes for the 16 x 16 tile): no corner force
-// ever goes to HBM.
+// (17x15x(cz+1) per 16x14xcz nodes for an inner 17 x 15 tile): no corner
+// force ever goes to HBM.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
-  // XCD-aware order for one domain: the dispatcher deals blocks round-robin
-  // over the 8 XCDs, so give each XCD a contiguous run of tiles (their halos
-  // share its L2; -8 % at s=192).  A batch keeps the plain order, which
-  // spreads every instance's workgroups -- of unequal cost -- over all XCDs.
-  const int nblocks = B.gblocks;
-  const int per = (nblocks + 7) / 8;
-  const int vb = B.n == 1 ? (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8) : (int)blockIdx.x;
-  if (vb >= nblocks) return;
+  // XCD-aware order: the dispatcher deals workgroups round-robin over the 8
+  // XCDs, and the host's table gives each XCD a contiguous run of blocks
+  // (neighbouring tiles, whose halos then share its L2; -8 % at s=192)
+  const int vb = B.gblock_order[blockIdx.x];
+  if (vb < 0) return;
   const int inst_ = B.gblock_inst[vb];
   const InstDesc I = B.inst[inst_];
   DevScalars *sc = B.sc + inst_;
@@ -348,23 +345,33 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
   const DevFields f = slice(B.f, I);
   const int lb = vb - I.gb0;
   const int tx = lb % I.tx, ty = (lb / I.tx) % I.ty, chunk = lb / (I.tx * I.ty);
-  const int i0 = tx * (etx - 1), j0 = ty * (ety - 1);
+  // the tile's first node column (the first tile owns etx columns, inner
+  // ones etx-1) and how far its first element lane sits before it (none
+  // before node 0)
+  const int i0 = tx == 0 ? 0 : etx + (tx - 1) * (etx - 1);
+  const int j0 = ty == 0 ? 0 : ety + (ty - 1) * (ety - 1);
+  const int oi = tx != 0, oj = ty != 0;
+  const int iend = tx == I.tx - 1 ? s + 1 : i0 + etx - oi;
+  const int jend = ty == I.ty - 1 ? s + 1 : j0 + ety - oj;
   const int k0 = chunk * I.cz;
   const int k1 = min(k0 + I.cz, s + 1);
   const int t = threadIdx.x, ti = t % etx, tj = t / etx;
   const int s1 = s + 1;
   // this thread's element in each plane, and its node column (threads past
   // the etx x ety tile take part in the barriers only)
-  const int ei = i0 - 1 + ti, ej = j0 - 1 + tj;
-  const bool elem_col = tj < ety && ei >= 0 && ei < s && ej >= 0 && ej < s;
+  const int ei = i0 - oi + ti, ej = j0 - oj + tj;
+  const bool elem_col = tj < ety && ei < s && ej < s;
   const int ni = i0 + ti, nj = j0 + tj;
-  const bool node_col = ti < etx - 1 && tj < ety - 1 && ni <= s && nj <= s;
+  const bool node_col = tj < ety && ni < iend && nj < jend;
   // the four elements around the node column, in ascending id order:
-  // (ni-1,nj-1) (ni,nj-1) (ni-1,nj) (ni,nj); LDS slot and node position
-  // inside each (corner = ox + 2oy... mapped to the hex order below)
+  // (ni-1,nj-1) (ni,nj-1) (ni-1,nj) (ni,nj) at lanes (ti+oi-1, tj+oj-1) ..
+  // (ti+oi, tj+oj); each is read only where it exists, which keeps every
+  // read inside the tile (a node column the tile owns has its elements
+  // among the tile's lanes)
   const bool h00 = ni >= 1 && nj >= 1, h10 = ni < s && nj >= 1;
   const bool h01 = ni >= 1 && nj < s, h11 = ni < s && nj < s;
-  const int l00 = tj * etx + ti, l10 = l00 + 1, l01 = l00 + etx, l11 = l00 + etx + 1;
+  const int l00 = (tj + oj - 1) * etx + (ti + oi - 1), l10 = l00 + 1, l01 = l00 + etx,
+            l11 = l00 + etx + 1;
   // corner of the node in element (ox,oy) = node offset inside it:
   // (1,1)->2, (0,1)->3, (1,0)->1, (0,0)->0 at the bottom; +4 at the top
 
@@ -1027,8 +1034,8 @@ hipError_t launch_math(hipStream_t st, int op, const double *a, const double *b,
 hipError_t launch_kernel(int id, hipStream_t st, const Batch &b) {
   switch (id) {
     case KID_NODAL:
-      // padded to a multiple of 8 for the XCD-aware block order
-      hipLaunchKernelGGL(k_nodal, dim3(((b.gblocks + 7) / 8) * 8), dim3(BLOCK), 0, st, b);
+      // glaunch workgroups, dealt to blocks by the XCD-aware order table
+      hipLaunchKernelGGL(k_nodal, dim3(b.glaunch), dim3(BLOCK), 0, st, b);
       break;
     case KID_KIN:
       hipLaunchKernelGGL(k_kin, dim3(b.eblocks), dim3(BLOCK), 0, st, b);
