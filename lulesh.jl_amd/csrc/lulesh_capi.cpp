@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "lulesh_adjoint.h"
@@ -315,7 +316,9 @@ int auto_depth(const lulesh_ctx *c) {
 // own L2.  XCD x gets the x-th of eight contiguous runs of the block order
 // (instance, chunk, tile row, tile), cut at equal sums of marched element
 // planes: neighbouring tiles, which read each other's halo nodes, share an
-// L2, and every XCD gets the same work.  `live` (null: all) restricts the
+// L2, and every XCD gets the same work.  Within a run the deepest chunks go
+// first (stable, so an instance's blocks stay together): the shallow ones
+// fill the launch's last round.  `live` (null: all) restricts the
 // table to the instances still running, so finished ones leave no hole in
 // one XCD's share.  The first call for a plan (glaunch 0) sizes the launch;
 // a later one that would need more workgroups than it has lists the live
@@ -346,8 +349,12 @@ int order_nodal(lulesh_ctx *c, const std::vector<char> *live) {
   std::vector<int> &order = c->gorder_host;
   order.assign((size_t)c->b.glaunch, -1);
   if (runlen * 8 <= c->b.glaunch) {
-    for (int r = 0; r < 8; ++r)
+    for (int r = 0; r < 8; ++r) {
+      std::stable_sort(lv.begin() + cut[r], lv.begin() + cut[r + 1], [&](int a, int b) {
+        return c->inst[gi[a]].cz > c->inst[gi[b]].cz;
+      });
       for (int k = 0; k < cut[r + 1] - cut[r]; ++k) order[(size_t)k * 8 + r] = lv[cut[r] + k];
+    }
   } else {
     for (int k = 0; k < nl; ++k) order[k] = lv[k];
   }
