@@ -46,12 +46,15 @@ def alg_bytes(E, N):
     }
 
 
-def bench_adjoint(nx=48, cycles=200, every=20):
+def bench_adjoint(nx=48, cycles=200, every=20, snapshots=10):
     """Adjoint/forward wall-time ratio: the gradient of the total internal
     energy after `cycles` cycles (hand-written adjoint kernels, checkpoints
     every `every` cycles, including its own forward pass) over the plain
-    forward run of the same cycles, both from the initial state."""
-    from lulesh_amd import Domain, LuleshProblem, grad_run, run
+    forward run of the same cycles, both from the initial state.  Beside it
+    the memory-bounded binomial schedule with `snapshots` saved states (same
+    gradient, bit for bit), with the adjoint memory of each scheme."""
+    import numpy as np
+    from lulesh_amd import Domain, LuleshProblem, grad_run, grad_run_revolve, run
     d = Domain(LuleshProblem(cycles, True, nx, 11, 1, 1))
     # warm-up: the forward graph is captured and the gradient's checkpoint and
     # segment buffers are allocated (first use) outside the timed runs
@@ -66,10 +69,23 @@ def bench_adjoint(nx=48, cycles=200, every=20):
     t0 = time.perf_counter()
     g = grad_run(d, cycles, every)
     grad = time.perf_counter() - t0
+    d.reset()
+    t0 = time.perf_counter()
+    gr = grad_run_revolve(d, cycles, snapshots)
+    grad_rv = time.perf_counter() - t0
+    E, N = d.numElem, d.numNode
     d.close()
+    state, slot = 8 * (6 * N + 5 * E), 8 * (6 * N + 14 * E)
+    mem_two = -(-cycles // every) * state + (every + 1) * slot
+    mem_rv = min(snapshots, cycles) * state + 2 * slot
     return {"config": f"s={nx}, {cycles} cycles, checkpoint every {every}",
             "forward_ms": round(fwd * 1e3, 3), "gradient_ms": round(grad * 1e3, 3),
-            "ratio": round(grad / fwd, 3), "loss": g["loss"]}
+            "ratio": round(grad / fwd, 3), "loss": g["loss"],
+            "checkpoint_MB": round(mem_two / 1e6, 1),
+            "revolve": {"snapshots": snapshots, "gradient_ms": round(grad_rv * 1e3, 3),
+                        "ratio": round(grad_rv / fwd, 3), "cycles_run": gr["forward_cycles"],
+                        "checkpoint_MB": round(mem_rv / 1e6, 1),
+                        "same_gradient": bool(np.array_equal(gr["grad_e"], g["grad_e"]))}}
 
 
 def bench_batched(n=64, seed=0, cap=5000):

@@ -169,6 +169,24 @@ int lulesh_batch_verify(lulesh_ctx *ctx, int32_t inst, double out[4]);
  * instead of timeIncrement! (for finite-difference checks of the gradient). */
 int lulesh_grad_run(lulesh_ctx *ctx, int64_t ncycles, int64_t checkpoint_every, double *grad_e,
                     double *grad_params, double *loss, double *dt_schedule);
+/* lulesh_grad_run in bounded memory (the same Enzyme call,
+ * examples/benchmark.jl:48-51,85-86, over many cycles): binomial
+ * checkpointing, Griewank & Walther's "revolve" (ACM TOMS Algorithm 799),
+ * keeps at most `snapshots` saved states (6N + 5E doubles each, the start
+ * state included) and two cycle slots (6N + 14E each) whatever ncycles is,
+ * and re-runs cycles as the optimal binomial schedule requires: with
+ * c = min(snapshots, ncycles) and r the least integer with
+ * C(c + r, c) >= ncycles, ncycles + r*ncycles - C(c + r, c + 1) cycles
+ * in all (the first pass is the schedule's first descent, each cycle's
+ * last run keeps its temporaries for the adjoint).  Outputs are those of
+ * lulesh_grad_run, bit for bit; *forward_cycles (may be NULL) returns the
+ * cycles run in all. */
+int lulesh_grad_run_revolve(lulesh_ctx *ctx, int64_t ncycles, int64_t snapshots, double *grad_e,
+                            double *grad_params, double *loss, double *dt_schedule,
+                            int64_t *forward_cycles);
+/* the cycles lulesh_grad_run_revolve runs for (ncycles, snapshots), first
+ * pass included (host-only; -1 for arguments < 1) */
+int64_t lulesh_revolve_cycles(int64_t ncycles, int64_t snapshots);
 int lulesh_replay(lulesh_ctx *ctx, const double *dt, int64_t ncycles);
 /* lulesh_vjp_cycle is the reference's per-cycle call
  * Enzyme.autodiff(lagrangeLeapFrog, Duplicated(domain, shadow))

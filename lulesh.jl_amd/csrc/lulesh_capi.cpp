@@ -669,7 +669,10 @@ int64_t cycle_slot(const lulesh_ctx *c) {
 
 // dt schedule of `ncycles`; with `every` > 0 also the adjoint fields, the
 // checkpoints and one segment of `every` cycle slots
-int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
+// Adjoint buffers for `ncycles` recorded time steps, `nstates` saved
+// states (6N + 5E doubles each) and a segment of `seg` cycle slots plus one
+// (seg <= 0: the time-step record only).
+int adj_setup_n(lulesh_ctx *c, int64_t ncycles, int64_t nstates, int64_t seg) {
   AdjBuffers &B = c->adj;
   const int64_t E = c->inst[0].E, N = c->inst[0].N;
   const int64_t state = 6 * N + 5 * E;
@@ -679,7 +682,7 @@ int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
     if (rc) return rc;
     B.dt_cap = ncycles;
   }
-  if (every <= 0) return 0;
+  if (seg <= 0) return 0;
   if (!B.ready) {
     AdjArgs &a = B.a;
     a.sc = c->b.sc;
@@ -717,18 +720,22 @@ int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
     for (double *p : zeroe) HIP_TRY(c, hipMemset(p, 0, sizeof(double) * E));
     B.ready = true;
   }
-  const int64_t nck = (ncycles + every - 1) / every;
-  if (nck > B.ckpt_cap) {
-    rc = adj_regrow(c, &B.ckpt, nck * state);
+  if (nstates > B.ckpt_cap) {
+    rc = adj_regrow(c, &B.ckpt, nstates * state);
     if (rc) return rc;
-    B.ckpt_cap = nck;
+    B.ckpt_cap = nstates;
   }
-  if (every > B.seg_cap) {
-    rc = adj_regrow(c, &B.seg, (every + 1) * cycle_slot(c));
+  if (seg > B.seg_cap) {
+    rc = adj_regrow(c, &B.seg, (seg + 1) * cycle_slot(c));
     if (rc) return rc;
-    B.seg_cap = every;
+    B.seg_cap = seg;
   }
   return 0;
+}
+
+// checkpoints every `every` cycles and a segment of `every` slots
+int adj_setup(lulesh_ctx *c, int64_t ncycles, int64_t every) {
+  return adj_setup_n(c, ncycles, every > 0 ? (ncycles + every - 1) / every : 0, every);
 }
 
 int state_copy(lulesh_ctx *c, double *buf, int to_buf) {
@@ -796,6 +803,70 @@ int adjoint_cycle(lulesh_ctx *c, const AdjArgs &aa, bool last) {
   return 0;
 }
 
+// The gradient run's forward pass: `ncycles` cycles of the driver loop from
+// the current state with timeIncrement! recording each dt; the start state
+// of cycle k is saved into save(k) when that is not null.  Leaves the loss
+// (sum_e e * elemMass, summed on the host in element order), the end state
+// in final_state/final_sc and the dt schedule in B.dt (and dt_out).
+template <typename Save>
+int grad_forward(lulesh_ctx *c, int64_t ncycles, long long cycle0, Save save, double *loss,
+                 double *dt_out) {
+  AdjBuffers &B = c->adj;
+  const int64_t E = c->inst[0].E;
+  const Batch a = make_args(c, 0, 0);
+  Batch ta = a;  // timeIncrement! that also records the dt it takes
+  ta.dt_record = B.dt;
+  ta.dt_base = cycle0;
+  int rc = clear_err(c);
+  for (int64_t k = 0; k < ncycles && !rc; ++k) {
+    double *buf = save(k);
+    if (buf) rc = state_copy(c, buf, 1);
+    if (!rc) rc = launch(c, KID_TIME_INCREMENT, ta);
+    const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
+    for (int id : ids)
+      if (!rc) rc = launch(c, id, a);
+  }
+  if (!rc) rc = finish(c, true);
+  if (rc) return rc;
+  if (dt_out) HIP_TRY(c, hipMemcpy(dt_out, B.dt, sizeof(double) * ncycles, hipMemcpyDeviceToHost));
+  {
+    std::vector<double> ev(E), mv(E);
+    HIP_TRY(c, hipMemcpy(ev.data(), c->b.f.e, sizeof(double) * E, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(mv.data(), c->b.f.elemMass, sizeof(double) * E, hipMemcpyDeviceToHost));
+    double L = 0.0;
+    for (int64_t i = 0; i < E; ++i) L += ev[i] * mv[i];
+    if (loss) *loss = L;
+  }
+  rc = state_copy(c, B.final_state, 1);
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(B.final_sc, c->b.sc, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream));
+  hipError_t e = launch_adjoint(AK_SEED, c->stream, B.a);
+  if (e != hipSuccess) return hip_fail(c, e, "adjoint seed");
+  return 0;
+}
+
+// gradients out, then the state the forward pass ended in
+int grad_results(lulesh_ctx *c, double *grad_e, double grad_params[3]) {
+  AdjBuffers &B = c->adj;
+  const int64_t E = c->inst[0].E;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (grad_e) HIP_TRY(c, hipMemcpy(grad_e, B.a.le, sizeof(double) * E, hipMemcpyDeviceToHost));
+  if (grad_params) {
+    std::vector<double> pa(3 * E);
+    HIP_TRY(c, hipMemcpy(pa.data(), B.a.pacc, sizeof(double) * 3 * E, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) {
+      double sum = 0.0;
+      for (int64_t i = 0; i < E; ++i) sum += pa[k * E + i];
+      grad_params[k] = sum;
+    }
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->b.sc, B.final_sc, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream));
+  int rc = state_copy(c, B.final_state, 0);
+  if (!rc) rc = finish(c, false);
+  c->partials_valid = false;
+  return rc;
+}
+
 int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, double grad_params[3],
              double *loss, double *dt_out) {
   if (c->n != 1) return fail(c, LULESH_ERR_UNSUPPORTED, "the adjoint needs a one-domain context");
@@ -809,43 +880,17 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
   if (!rc) rc = finish(c, false);
   if (rc) return rc;
   const long long cycle0 = c->hsc[0].cycle;
-  const Batch a = make_args(c, 0, 0);
-  Batch ta = a;  // timeIncrement! that also records the dt it takes
-  ta.dt_record = B.dt;
-  ta.dt_base = cycle0;
   // 1. forward pass: checkpoints every `every` cycles, dt schedule recorded
-  rc = clear_err(c);
-  for (int64_t k = 0; k < ncycles && !rc; ++k) {
-    if (k % every == 0) rc = state_copy(c, B.ckpt + (k / every) * state, 1);
-    if (!rc) rc = launch(c, KID_TIME_INCREMENT, ta);
-    const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
-    for (int id : ids)
-      if (!rc) rc = launch(c, id, a);
-  }
-  if (!rc) rc = finish(c, true);
+  rc = grad_forward(
+      c, ncycles, cycle0,
+      [&](int64_t k) -> double * { return k % every == 0 ? B.ckpt + (k / every) * state : nullptr; },
+      loss, dt_out);
   if (rc) return rc;
-  if (dt_out) HIP_TRY(c, hipMemcpy(dt_out, B.dt, sizeof(double) * ncycles, hipMemcpyDeviceToHost));
-  // loss = sum_e e * elemMass, summed on the host in element order
-  {
-    std::vector<double> ev(E), mv(E);
-    HIP_TRY(c, hipMemcpy(ev.data(), c->b.f.e, sizeof(double) * E, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(mv.data(), c->b.f.elemMass, sizeof(double) * E, hipMemcpyDeviceToHost));
-    double L = 0.0;
-    for (int64_t i = 0; i < E; ++i) L += ev[i] * mv[i];
-    if (loss) *loss = L;
-  }
-  rc = state_copy(c, B.final_state, 1);
-  if (rc) return rc;
-  HIP_TRY(c, hipMemcpyAsync(B.final_sc, c->b.sc, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream));
   // 2. reverse sweep, segment by segment: restore the segment's checkpoint,
   // replay it storing each cycle's start state and temporaries, then run
   // the adjoint kernels of its cycles backwards against the stored slots
   std::vector<double> hdt(ncycles);
   HIP_TRY(c, hipMemcpy(hdt.data(), B.dt, sizeof(double) * ncycles, hipMemcpyDeviceToHost));
-  {
-    hipError_t e = launch_adjoint(AK_SEED, c->stream, B.a);
-    if (e != hipSuccess) return hip_fail(c, e, "adjoint seed");
-  }
   const int64_t nseg = (ncycles + every - 1) / every;
   for (int64_t sg = nseg - 1; sg >= 0 && !rc; --sg) {
     const int64_t k0 = sg * every, k1 = (k0 + every < ncycles) ? k0 + every : ncycles;
@@ -871,22 +916,208 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
   }
   if (rc) return rc;
   // 3. results, then the state the forward pass ended in
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  if (grad_e) HIP_TRY(c, hipMemcpy(grad_e, B.a.le, sizeof(double) * E, hipMemcpyDeviceToHost));
-  if (grad_params) {
-    std::vector<double> pa(3 * E);
-    HIP_TRY(c, hipMemcpy(pa.data(), B.a.pacc, sizeof(double) * 3 * E, hipMemcpyDeviceToHost));
-    for (int k = 0; k < 3; ++k) {
-      double sum = 0.0;
-      for (int64_t i = 0; i < E; ++i) sum += pa[k * E + i];
-      grad_params[k] = sum;
-    }
+  return grad_results(c, grad_e, grad_params);
+}
+
+// ---------------------------------------------------------------------------
+// binomial checkpointing: Griewank, "Achieving logarithmic growth of temporal
+// and spatial complexity in reverse automatic differentiation" (Optimization
+// Methods and Software 1, 1992) and Griewank & Walther, "Algorithm 799:
+// revolve" (ACM TOMS 26, 2000).  With c saved states (the start one
+// included), reversing n steps needs at least
+//   t(n, c) = r n - beta(c + 1, r - 1)   forward steps besides the n
+// recording ones, r the least integer with beta(c, r) >= n,
+// beta(c, r) = C(c + r, c).
+// ---------------------------------------------------------------------------
+const int64_t kSat = (int64_t)1 << 62;
+
+int64_t beta_sat(int64_t c, int64_t r) {
+  if (r < 0) return 0;
+  unsigned __int128 v = 1;  // C(c + i, i) for i = 0..r, each step exact
+  for (int64_t i = 1; i <= r; ++i) {
+    v = v * (unsigned __int128)(c + i) / (unsigned __int128)i;
+    if (v >= (unsigned __int128)kSat) return kSat;
   }
-  HIP_TRY(c, hipMemcpyAsync(c->b.sc, B.final_sc, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream));
-  rc = state_copy(c, B.final_state, 0);
+  return (int64_t)v;
+}
+
+// fewest advancing forward steps that reverse n steps from a saved start
+// state with `spare` further states free
+int64_t revolve_cost(int64_t n, int64_t spare) {
+  if (n <= 1) return 0;
+  if (spare == 0) return n * (n - 1) / 2;
+  const int64_t c = spare + 1;
+  int64_t r = 0;
+  while (beta_sat(c, r) < n) ++r;
+  return r * n - beta_sat(c + 1, r - 1);
+}
+
+// where to save the next state: advance l steps (1 <= l < n).  The total
+// l + cost(n - l, spare - 1) + cost(l, spare) is convex in l (the cost is
+// piecewise linear in n with slopes r, non-decreasing), so the first l whose
+// forward difference is >= 0 minimises it.
+int64_t revolve_split(int64_t n, int64_t spare) {
+  auto g = [&](int64_t l) { return l + revolve_cost(n - l, spare - 1) + revolve_cost(l, spare); };
+  int64_t lo = 1, hi = n - 1;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (g(mid + 1) - g(mid) >= 0) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+struct Revolve {
+  lulesh_ctx *c;
+  int64_t n;
+  long long cycle0;
+  Batch a, ta;               // the cycle; timeIncrement! recording its dt
+  std::vector<int64_t> pos;  // cycle whose start state each buffer holds (-1: free)
+  std::vector<double> hdt;
+  int64_t frontier = 0;      // first cycle not run yet (its dt not recorded)
+  int64_t cur = -1;          // cycle whose start state the context holds (-1: other)
+  int64_t forward = 0;       // cycles run
+  double *loss = nullptr;
+  double *buf(int i) {
+    const int64_t E = c->inst[0].E, N = c->inst[0].N;
+    return c->adj.ckpt + (int64_t)i * (6 * N + 5 * E);
+  }
+  int find(int64_t k) const {
+    for (size_t i = 0; i < pos.size(); ++i)
+      if (pos[i] == k) return (int)i;
+    return -1;
+  }
+  int restore(int64_t k) {
+    if (cur == k) return 0;
+    const int i = find(k);
+    if (i < 0) return fail(c, LULESH_ERR_HIP, "revolve: missing saved state");
+    int rc = state_copy(c, buf(i), 0);
+    if (rc) return rc;
+    hipError_t e = launch_set_cycle(c->stream, c->b.sc, cycle0 + k);
+    if (e != hipSuccess) return hip_fail(c, e, "set cycle");
+    cur = k;
+    return 0;
+  }
+  // one cycle from the current state: the first time with timeIncrement!
+  // recording its dt and the time constraints, after that at the recorded
+  // dt (state_slot: where to keep the start state, or null)
+  int step(int64_t k, double *state_slot) {
+    int rc = 0;
+    if (k == frontier) {
+      if (state_slot) rc = slot_step(c, nullptr, state_slot, nullptr, 0);
+      if (!rc) rc = launch(c, KID_TIME_INCREMENT, ta);
+      const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
+      for (int id : ids)
+        if (!rc) rc = launch(c, id, a);
+      ++frontier;
+    } else {
+      rc = slot_step(c, nullptr, state_slot, c->adj.dt, cycle0);
+      if (!rc) rc = cycle_body(c, false);
+    }
+    ++forward;
+    return rc;
+  }
+  int advance(int64_t from, int64_t to) {
+    int rc = restore(from);
+    for (int64_t k = from; k < to && !rc; ++k) rc = step(k, nullptr);
+    cur = to;
+    return rc;
+  }
+  int save(int64_t k) {
+    const int i = find(-1);
+    if (i < 0) return fail(c, LULESH_ERR_HIP, "revolve: no free state buffer");
+    pos[i] = k;
+    return state_copy(c, buf(i), 1);
+  }
+  // the forward pass has reached the end: errors, loss, end state, dt record
+  int end_of_forward() {
+    AdjBuffers &B = c->adj;
+    const int64_t E = c->inst[0].E;
+    int rc = finish(c, true);
+    if (rc) return rc;
+    std::vector<double> ev(E), mv(E);
+    HIP_TRY(c, hipMemcpy(ev.data(), c->b.f.e, sizeof(double) * E, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(mv.data(), c->b.f.elemMass, sizeof(double) * E, hipMemcpyDeviceToHost));
+    double L = 0.0;
+    for (int64_t i = 0; i < E; ++i) L += ev[i] * mv[i];
+    if (loss) *loss = L;
+    rc = state_copy(c, B.final_state, 1);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(B.final_sc, c->b.sc, sizeof(DevScalars), hipMemcpyDeviceToDevice,
+                              c->stream));
+    HIP_TRY(c, hipMemcpy(hdt.data(), B.dt, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return 0;
+  }
+  // cycle k forward with its start state and temporaries kept, then its
+  // adjoint kernels
+  int youturn(int64_t k) {
+    int rc = restore(k);
+    double *sa = seg_slot(c, 0), *sb = seg_slot(c, 1);
+    const bool last = k == frontier;  // the forward pass's last cycle
+    if (!rc) rc = step(k, sa);
+    if (!rc) rc = slot_step(c, slot_temps(c, sa), sb, nullptr, 0);
+    if (!rc && last) rc = end_of_forward();
+    if (!rc) rc = adjoint_cycle(c, stored_args(c, sa, sb, hdt[k], k == n - 1 ? 0 : 1), k == 0);
+    cur = -1;
+    return rc;
+  }
+  // reverse cycles b-1 .. a, the start state of cycle a saved, `spare`
+  // buffers free
+  int reverse(int64_t a0, int64_t b, int64_t spare) {
+    if (b - a0 == 1) return youturn(a0);
+    if (spare == 0) {
+      for (int64_t k = b - 1; k >= a0; --k) {
+        int rc = advance(a0, k);
+        if (!rc) rc = youturn(k);
+        if (rc) return rc;
+      }
+      return 0;
+    }
+    const int64_t m = a0 + revolve_split(b - a0, spare);
+    int rc = advance(a0, m);
+    if (!rc) rc = save(m);
+    if (!rc) rc = reverse(m, b, spare - 1);
+    if (rc) return rc;
+    pos[find(m)] = -1;
+    return reverse(a0, m, spare);
+  }
+};
+
+int grad_run_revolve(lulesh_ctx *c, int64_t ncycles, int64_t snapshots, double *grad_e,
+                     double grad_params[3], double *loss, double *dt_out, int64_t *forward_cycles) {
+  if (c->n != 1) return fail(c, LULESH_ERR_UNSUPPORTED, "the adjoint needs a one-domain context");
+  if (ncycles < 1 || snapshots < 1)
+    return fail(c, LULESH_ERR_INVALID_ARG, "ncycles and snapshots must be >= 1");
+  if (snapshots > ncycles) snapshots = ncycles;
+  HIP_TRY(c, hipSetDevice(c->device));
+  int rc = adj_setup_n(c, ncycles, snapshots, 1);
+  if (rc) return rc;
+  rc = pull_scalars(c);
   if (!rc) rc = finish(c, false);
-  c->partials_valid = false;
-  return rc;
+  if (rc) return rc;
+  Revolve R;
+  R.c = c;
+  R.n = ncycles;
+  R.cycle0 = c->hsc[0].cycle;
+  R.a = make_args(c, 0, 0);
+  R.ta = R.a;
+  R.ta.dt_record = c->adj.dt;
+  R.ta.dt_base = R.cycle0;
+  R.pos.assign((size_t)snapshots, -1);
+  R.hdt.resize(ncycles);
+  R.loss = loss;
+  rc = clear_err(c);
+  if (!rc) {
+    hipError_t e = launch_adjoint(AK_SEED, c->stream, c->adj.a);
+    if (e != hipSuccess) rc = hip_fail(c, e, "adjoint seed");
+  }
+  if (!rc) rc = R.save(0);
+  R.cur = 0;
+  if (!rc) rc = R.reverse(0, ncycles, snapshots - 1);
+  if (rc) return rc;
+  if (dt_out) std::memcpy(dt_out, R.hdt.data(), sizeof(double) * ncycles);
+  if (forward_cycles) *forward_cycles = R.forward;
+  return grad_results(c, grad_e, grad_params);
 }
 
 // one lagrangeLeapFrog with the shadow mapped from the adjoint of the state
@@ -987,6 +1218,20 @@ int lulesh_grad_run(lulesh_ctx *c, int64_t ncycles, int64_t checkpoint_every, do
                     double *grad_params, double *loss, double *dt_schedule) {
   if (!c) return LULESH_ERR_INVALID_ARG;
   return grad_run(c, ncycles, checkpoint_every, grad_e, grad_params, loss, dt_schedule);
+}
+
+int lulesh_grad_run_revolve(lulesh_ctx *c, int64_t ncycles, int64_t snapshots, double *grad_e,
+                            double *grad_params, double *loss, double *dt_schedule,
+                            int64_t *forward_cycles) {
+  if (!c) return LULESH_ERR_INVALID_ARG;
+  return grad_run_revolve(c, ncycles, snapshots, grad_e, grad_params, loss, dt_schedule,
+                          forward_cycles);
+}
+
+int64_t lulesh_revolve_cycles(int64_t ncycles, int64_t snapshots) {
+  if (ncycles < 1 || snapshots < 1) return -1;
+  if (snapshots > ncycles) snapshots = ncycles;
+  return ncycles + revolve_cost(ncycles, snapshots - 1);
 }
 
 int lulesh_replay(lulesh_ctx *c, const double *dt, int64_t ncycles) {

@@ -65,6 +65,9 @@ SIGNATURES = {
     "lulesh_autotune": (c_int, [c_void_p, P_int32, P_double]),
     "lulesh_grad_run": (c_int, [c_void_p, c_int64, c_int64, P_double, P_double, P_double,
                                 P_double]),
+    "lulesh_grad_run_revolve": (c_int, [c_void_p, c_int64, c_int64, P_double, P_double, P_double,
+                                        P_double, P_int64]),
+    "lulesh_revolve_cycles": (c_int64, [c_int64, c_int64]),
     "lulesh_replay": (c_int, [c_void_p, P_double, c_int64]),
     "lulesh_vjp_cycle": (c_int, [c_void_p, P_double]),
     "lulesh_batch_error": (c_int, [c_void_p, c_int32]),

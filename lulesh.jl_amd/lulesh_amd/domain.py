@@ -281,6 +281,25 @@ def grad_run(domain, ncycles, checkpoint_every=20):
             "grad_qqc_monoq": gp[2], "dt": dt}
 
 
+def grad_run_revolve(domain, ncycles, snapshots):
+    """grad_run in bounded memory: binomial checkpointing (Griewank and
+    Walther's revolve) with at most `snapshots` saved states, re-running
+    cycles as the optimal schedule requires.  Same results as grad_run, bit
+    for bit, plus "forward_cycles", the cycles run in all."""
+    E = domain.numElem
+    ge = np.zeros(E)
+    gp = np.zeros(3)
+    loss = ctypes.c_double()
+    dt = np.zeros(int(ncycles))
+    fwd = ctypes.c_int64()
+    _check(domain._lib, domain._ctx, domain._lib.lulesh_grad_run_revolve(
+        domain._ctx, int(ncycles), int(snapshots), ge.ctypes.data_as(_lib.P_double),
+        gp.ctypes.data_as(_lib.P_double), ctypes.byref(loss), dt.ctypes.data_as(_lib.P_double),
+        ctypes.byref(fwd)))
+    return {"loss": loss.value, "grad_e": ge, "grad_hgcoef": gp[0], "grad_qlc_monoq": gp[1],
+            "grad_qqc_monoq": gp[2], "dt": dt, "forward_cycles": fwd.value}
+
+
 def replay(domain, dt):
     """Advance with a prescribed dt per cycle instead of timeIncrement!."""
     a = np.ascontiguousarray(dt, dtype=np.float64)
