@@ -117,6 +117,20 @@ LDEV NodeSet pick(const DevFields &f, int k) {
 // Per-block instance lookup for the element-, node- and k_nodal-shaped
 // grids: every block belongs to exactly one instance (instances start on a
 // block boundary), so the lookup is uniform over the block.
+// XCD-grouped block order for the element kernels: the dispatcher deals
+// workgroups round-robin over the 8 XCDs, so hand each XCD runs of G
+// consecutive blocks (rows of neighbouring elements that share node rows in
+// its L2) while the eight XCDs still sweep nearby addresses together.  The
+// launch is padded to a multiple of 8G; -1 marks a padding workgroup.
+template <unsigned G>
+__device__ __forceinline__ int xcd_group_block(unsigned b, int nb) {
+  const unsigned x = b % 8u, k = b / 8u;
+  const unsigned v = ((k / G) * 8u + x) * G + k % G;
+  return v < (unsigned)nb ? (int)v : -1;
+}
+constexpr unsigned KIN_GROUP = 16, QEOS_GROUP = 4;
+inline unsigned grouped_grid(int nb, unsigned g) { return ((unsigned)nb + 8 * g - 1) / (8 * g) * (8 * g); }
+
 #define LULESH_ELEM_PREAMBLE(B)                                                  \
   const int inst_ = (B).eblock_inst[blockIdx.x];                               \
   const InstDesc I = (B).inst[inst_];                                          \
@@ -124,6 +138,16 @@ LDEV NodeSet pick(const DevFields &f, int k) {
   const int s = I.s;                                                           \
   const int64_t E = I.E;                                                       \
   const int64_t e = (int64_t)(blockIdx.x - I.eb0) * BLOCK + threadIdx.x;       \
+  const DevFields f = slice((B).f, I);
+#define LULESH_ELEM_PREAMBLE_G(B, G)                                             \
+  const int vb_ = xcd_group_block<G>(blockIdx.x, (B).eblocks);                 \
+  if (vb_ < 0) return;                                                         \
+  const int inst_ = (B).eblock_inst[vb_];                                      \
+  const InstDesc I = (B).inst[inst_];                                          \
+  DevScalars *sc = (B).sc + inst_;                                             \
+  const int s = I.s;                                                           \
+  const int64_t E = I.E;                                                       \
+  const int64_t e = (int64_t)(vb_ - I.eb0) * BLOCK + threadIdx.x;              \
   const DevFields f = slice((B).f, I);
 #define LULESH_NODE_PREAMBLE(B)                                                  \
   const int inst_ = (B).nblock_inst[blockIdx.x];                               \
@@ -589,7 +613,7 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
 // calcKinematicsForElems (src/domain.jl:2099-2147), the deviatoric loop
 // (2156-2171) and calcMonotonicQGradientsForElems (2175-2320)
 __global__ __launch_bounds__(BLOCK) void k_kin(Batch B) {
-  LULESH_ELEM_PREAMBLE(B)
+  LULESH_ELEM_PREAMBLE_G(B, KIN_GROUP)
   if (e >= E) return;
   if (B.gated && !sc->active) return;
   const int keep_aux = B.keep_aux;
@@ -721,7 +745,7 @@ LDEV void block_min2(double &a, double &b) {
 // updateVolumesForElems (2912-2930) and the per-element terms of
 // calcCourantConstraintForElems / calcHydroConstraintForElems (2964-3054)
 __global__ __launch_bounds__(BLOCK) void k_qeos(Batch B) {
-  LULESH_ELEM_PREAMBLE(B)
+  LULESH_ELEM_PREAMBLE_G(B, QEOS_GROUP)
   if (B.gated && !sc->active) return;  // uniform over the block
   const int quirks = B.quirks, keep_aux = B.keep_aux;
   double *__restrict__ dtpart = B.dtpart;
@@ -897,8 +921,8 @@ __global__ __launch_bounds__(BLOCK) void k_qeos(Batch B) {
   }
   block_min2(cand_c, cand_h);
   if (threadIdx.x == 0) {
-    dtpart[2 * blockIdx.x] = cand_c;
-    dtpart[2 * blockIdx.x + 1] = cand_h;
+    dtpart[2 * vb_] = cand_c;
+    dtpart[2 * vb_ + 1] = cand_h;
   }
 }
 
@@ -1038,10 +1062,10 @@ hipError_t launch_kernel(int id, hipStream_t st, const Batch &b) {
       hipLaunchKernelGGL(k_nodal, dim3(b.glaunch), dim3(BLOCK), 0, st, b);
       break;
     case KID_KIN:
-      hipLaunchKernelGGL(k_kin, dim3(b.eblocks), dim3(BLOCK), 0, st, b);
+      hipLaunchKernelGGL(k_kin, dim3(grouped_grid(b.eblocks, KIN_GROUP)), dim3(BLOCK), 0, st, b);
       break;
     case KID_QEOS:
-      hipLaunchKernelGGL(k_qeos, dim3(b.eblocks), dim3(BLOCK), 0, st, b);
+      hipLaunchKernelGGL(k_qeos, dim3(grouped_grid(b.eblocks, QEOS_GROUP)), dim3(BLOCK), 0, st, b);
       break;
     case KID_DT:
       hipLaunchKernelGGL(k_dt, dim3(b.n), dim3(1024), 0, st, b);
