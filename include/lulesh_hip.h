@@ -219,13 +219,21 @@ int lulesh_set_timing(lulesh_ctx *ctx, int32_t on);
  * `depth` node planes of its node column (16x14 nodes for large domains).
  * 0 restores the default: per instance, node planes split into equal
  * chunks whose depth minimises (launch rounds x planes per workgroup), a
- * round being CUs x the workgroups one CU holds.  lulesh_autotune times
- * k_nodal on the live state for the default and depths 2..32 (with its
- * parity flip off the kernel only writes the inactive node set, so the
- * state is unchanged) and keeps the fastest; *chosen (the depth of
- * instance 0) and *best_ms (per launch) may be NULL. */
+ * round being CUs x the workgroups one CU holds.  Tile shape: per
+ * instance the etx x (256/etx) element tile, etx in [lo, hi], that covers
+ * the node columns with the fewest tiles (lulesh_set_nodal_tiles; 0, 0
+ * restores the default 16..32; 2 <= lo <= hi <= 128); lulesh_nodal_tile
+ * reports an instance's {etx, ety, tiles along x, tiles along y}.  Neither
+ * changes any result: every node sums its corners in the same order.
+ * lulesh_autotune times k_nodal on the live state for five tile-width
+ * ranges x the default and depths 2..32 (with its parity flip off the
+ * kernel only writes the inactive node set, so the state is unchanged) and
+ * keeps the fastest pair; *chosen (the depth of instance 0) and *best_ms
+ * (per launch) may be NULL. */
 int lulesh_set_nodal_depth(lulesh_ctx *ctx, int32_t depth);
 int32_t lulesh_nodal_depth(const lulesh_ctx *ctx);
+int lulesh_set_nodal_tiles(lulesh_ctx *ctx, int32_t lo, int32_t hi);
+int lulesh_nodal_tile(const lulesh_ctx *ctx, int32_t inst, int32_t *shape);
 int lulesh_autotune(lulesh_ctx *ctx, int32_t *chosen, double *best_ms);
 /* Device arithmetic check (no reference counterpart): evaluates, on device
  * `device`, the scalar routines every kernel relies on, so tests can pin

@@ -64,6 +64,10 @@ struct lulesh_ctx {
   int graph_keep_aux = -1;
   std::string last_error;
   std::vector<void *> allocations;
+  int tile_lo = 16, tile_hi = 32;  // k_nodal tile widths allowed (nodal_tile)
+  int depth_req = 0;               // k_nodal z-depth asked for (0: the model's)
+  int *ginst = nullptr;            // device table of gblock_host (regrown by plan_nodal)
+  int64_t ginst_cap = 0;
   std::vector<int> gblock_host;  // instance of each k_nodal block
   int *gorder = nullptr;         // k_nodal dispatch table (regrown by plan_nodal)
   int64_t gorder_cap = 0;
@@ -237,6 +241,7 @@ void release(lulesh_ctx *c) {
   for (void *p : c->allocations) (void)hipFree(p);
   for (void *p : c->adj.mem) (void)hipFree(p);
   if (c->gorder) (void)hipFree(c->gorder);
+  if (c->ginst) (void)hipFree(c->ginst);
   if (c->hsc) (void)hipHostFree(c->hsc);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -375,6 +380,13 @@ int order_nodal(lulesh_ctx *c, const std::vector<char> *live) {
 }
 
 int plan_nodal(lulesh_ctx *c, int cz) {
+  for (InstDesc &I : c->inst) {
+    const NodalTile t = nodal_tile(I.s, c->tile_lo, c->tile_hi);
+    I.etx = t.etx;
+    I.ety = t.ety;
+    I.tx = t.tx;
+    I.ty = t.ty;
+  }
   // default: auto_depth, balanced per instance; an explicit depth applies
   // unchanged to all instances
   const bool automatic = cz <= 0;
@@ -392,9 +404,18 @@ int plan_nodal(lulesh_ctx *c, int cz) {
   c->b.gblocks = gb;
   c->gblock_host = std::move(gblock_inst);
   c->b.glaunch = 0;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (gb > c->ginst_cap) {
+    if (c->ginst) HIP_TRY(c, hipFree(c->ginst));
+    c->ginst = nullptr;
+    c->ginst_cap = 0;
+    HIP_TRY(c, hipMalloc((void **)&c->ginst, sizeof(int) * gb));
+    c->ginst_cap = gb;
+  }
+  c->b.gblock_inst = c->ginst;
   HIP_TRY(c, hipMemcpyAsync((void *)c->b.inst, c->inst.data(), sizeof(InstDesc) * c->n,
                             hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(c, hipMemcpyAsync((void *)c->b.gblock_inst, c->gblock_host.data(), sizeof(int) * gb,
+  HIP_TRY(c, hipMemcpyAsync(c->ginst, c->gblock_host.data(), sizeof(int) * gb,
                             hipMemcpyHostToDevice, c->stream));
   const int rc = order_nodal(c, nullptr);
   if (rc) return rc;
@@ -438,7 +459,7 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
     c->nodal_per_cu = nodal_blocks_per_cu();
   }
   c->inst.resize(n);
-  int eb = 0, nb = 0, gb_max = 0;
+  int eb = 0, nb = 0;
   int64_t e0 = 0, n0 = 0;
   std::vector<int> eblock_inst, nblock_inst;
   for (int i = 0; i < n; ++i) {
@@ -467,7 +488,6 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
     n0 += I.N;
     eb += I.ebn;
     nb += I.nbn;
-    gb_max += nodal_blocks(tile, s, 1);
     eblock_inst.insert(eblock_inst.end(), I.ebn, i);
     nblock_inst.insert(nblock_inst.end(), I.nbn, i);
     c->nr.push_back(nr ? nr[i] : 1);
@@ -508,11 +528,10 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
   if (!rc) rc = dev_alloc(c, &B.dtpart, 2 * (int64_t)eb);
   if (!rc) rc = dev_alloc(c, &B.sc, n);
   InstDesc *dinst = nullptr;
-  int *deb = nullptr, *dnb = nullptr, *dgb = nullptr;
+  int *deb = nullptr, *dnb = nullptr;
   if (!rc) rc = dev_alloc(c, &dinst, n);
   if (!rc) rc = dev_alloc(c, &deb, eb);
   if (!rc) rc = dev_alloc(c, &dnb, nb);
-  if (!rc) rc = dev_alloc(c, &dgb, gb_max);
   if (!rc) {
     he = hipMemcpy(deb, eblock_inst.data(), sizeof(int) * eb, hipMemcpyHostToDevice);
     if (he == hipSuccess) he = hipMemcpy(dnb, nblock_inst.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
@@ -521,7 +540,6 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
   B.inst = dinst;
   B.eblock_inst = deb;
   B.nblock_inst = dnb;
-  B.gblock_inst = dgb;
   if (!rc) rc = plan_nodal(c, 0);
   if (!rc) rc = lulesh_reset(c);
   if (rc) {
@@ -1570,7 +1588,32 @@ int lulesh_math_check(int32_t op, const double *a, const double *b, double *out,
 int lulesh_set_nodal_depth(lulesh_ctx *c, int32_t cz) {
   if (!c || cz < 0 || cz > 1024) return LULESH_ERR_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
+  c->depth_req = cz;
   return plan_nodal(c, cz);
+}
+
+int lulesh_set_nodal_tiles(lulesh_ctx *c, int32_t lo, int32_t hi) {
+  if (!c) return LULESH_ERR_INVALID_ARG;
+  if (lo == 0 && hi == 0) {
+    lo = 16;
+    hi = 32;
+  }
+  if (lo < 2 || hi < lo || hi > BLOCK / 2)
+    return fail(c, LULESH_ERR_INVALID_ARG, "tile widths must satisfy 2 <= lo <= hi <= 128");
+  HIP_TRY(c, hipSetDevice(c->device));
+  c->tile_lo = lo;
+  c->tile_hi = hi;
+  return plan_nodal(c, c->depth_req);
+}
+
+int lulesh_nodal_tile(const lulesh_ctx *c, int32_t inst, int32_t *shape) {
+  if (!c || !shape || inst < 0 || inst >= c->n) return LULESH_ERR_INVALID_ARG;
+  const InstDesc &I = c->inst[inst];
+  shape[0] = I.etx;
+  shape[1] = I.ety;
+  shape[2] = I.tx;
+  shape[3] = I.ty;
+  return 0;
 }
 
 int32_t lulesh_nodal_depth(const lulesh_ctx *c) { return c ? c->inst[0].cz : -1; }
@@ -1580,42 +1623,52 @@ int lulesh_autotune(lulesh_ctx *c, int32_t *chosen, double *best_ms) {
   HIP_TRY(c, hipSetDevice(c->device));
   // k_nodal with its parity flip switched off reads the current node set
   // and writes only the other one, so it can be timed on the live state
-  // without changing it
-  const int cand[] = {0 /* the model's choice */, 2, 4, 8, 16, 32};
-  int best = -1;
+  // without changing it.  Candidates: tile-width ranges (the default first)
+  // x z-depths (the model's first).
+  const int fam[][2] = {{16, 32}, {16, 16}, {12, 21}, {8, 32}, {24, 32}};
+  const int cand[] = {0, 2, 4, 8, 16, 32};
+  int maxs = 0;
+  for (const InstDesc &I : c->inst) maxs = I.s > maxs ? I.s : maxs;
+  int best_f = -1, best_cz = -1;
   float best_t = 0.f;
   hipEvent_t ea = get_event(c), eb = get_event(c);
   if (!ea || !eb) return fail(c, LULESH_ERR_HIP, "hipEventCreate failed");
   int rc = 0;
-  for (int cz : cand) {
-    int maxs = 0;
-    for (const InstDesc &I : c->inst) maxs = I.s > maxs ? I.s : maxs;
-    if (cz > 2 && cz / 2 > maxs + 1) break;
-    rc = plan_nodal(c, cz);
-    if (rc) break;
-    Batch a = make_args(c, 0, 0);
-    a.flip = 0;
-    hipError_t e = hipSuccess;
-    for (int w = 0; w < 2 && e == hipSuccess; ++w) e = launch_kernel(KID_NODAL, c->stream, a);
-    if (e == hipSuccess) e = hipEventRecord(ea, c->stream);
-    for (int r = 0; r < 5 && e == hipSuccess; ++r) e = launch_kernel(KID_NODAL, c->stream, a);
-    if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(eb);
-    float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
-    if (e != hipSuccess) {
-      rc = hip_fail(c, e, "autotune");
-      break;
-    }
-    if (best < 0 || ms < best_t) {
-      best = cz;
-      best_t = ms;
+  for (int fi = 0; fi < (int)(sizeof(fam) / sizeof(fam[0])) && !rc; ++fi) {
+    c->tile_lo = fam[fi][0];
+    c->tile_hi = fam[fi][1];
+    for (int cz : cand) {
+      if (cz > 2 && cz / 2 > maxs + 1) break;
+      rc = plan_nodal(c, cz);
+      if (rc) break;
+      Batch a = make_args(c, 0, 0);
+      a.flip = 0;
+      hipError_t e = hipSuccess;
+      for (int w = 0; w < 2 && e == hipSuccess; ++w) e = launch_kernel(KID_NODAL, c->stream, a);
+      if (e == hipSuccess) e = hipEventRecord(ea, c->stream);
+      for (int r = 0; r < 5 && e == hipSuccess; ++r) e = launch_kernel(KID_NODAL, c->stream, a);
+      if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(eb);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
+      if (e != hipSuccess) {
+        rc = hip_fail(c, e, "autotune");
+        break;
+      }
+      if (best_f < 0 || ms < best_t) {
+        best_f = fi;
+        best_cz = cz;
+        best_t = ms;
+      }
     }
   }
   c->event_pool.push_back(ea);
   c->event_pool.push_back(eb);
   if (rc) return rc;
-  rc = plan_nodal(c, best);
+  c->tile_lo = fam[best_f][0];
+  c->tile_hi = fam[best_f][1];
+  c->depth_req = best_cz;
+  rc = plan_nodal(c, best_cz);
   if (!rc) rc = clear_err(c);
   if (!rc) rc = finish(c, false);
   if (rc) return rc;

@@ -114,13 +114,14 @@ inline int nodal_span(int s, int a) {
 struct NodalTile {
   int etx, ety, tx, ty;
 };
-// Per instance: the tile, etx in 16..32 lanes wide (whole cache-line runs
-// of a row), ety = BLOCK / etx, that covers the (s+1)^2 node columns with
-// the fewest tiles (the squarest among equals).  At s=192 that is 17 x 15
-// (12 x 14 tiles); small domains avoid a mostly idle last tile.
-inline NodalTile nodal_tile(int s) {
+// Per instance: the tile, etx in lo..hi lanes wide (by default 16..32: a
+// row's node loads span whole 128-byte lines), ety = BLOCK / etx, that
+// covers the (s+1)^2 node columns with the fewest tiles (the squarest among
+// equals).  At s=192 that is 17 x 15 (12 x 14 tiles); small domains avoid a
+// mostly idle last tile.  2 <= lo <= hi <= BLOCK / 2.
+inline NodalTile nodal_tile(int s, int lo = 16, int hi = 32) {
   NodalTile best{0, 0, 0, 0};
-  for (int a = 16; a <= 32; ++a) {
+  for (int a = lo; a <= hi; ++a) {
     const int b = BLOCK / a;
     const int tx = nodal_span(s, a), ty = nodal_span(s, b);
     const int dcand = a > b ? a - b : b - a;

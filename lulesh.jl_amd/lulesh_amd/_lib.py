@@ -62,6 +62,8 @@ SIGNATURES = {
     "lulesh_batch_verify": (c_int, [c_void_p, c_int32, P_double]),
     "lulesh_set_nodal_depth": (c_int, [c_void_p, c_int32]),
     "lulesh_nodal_depth": (c_int32, [c_void_p]),
+    "lulesh_set_nodal_tiles": (c_int, [c_void_p, c_int32, c_int32]),
+    "lulesh_nodal_tile": (c_int, [c_void_p, c_int32, P_int32]),
     "lulesh_autotune": (c_int, [c_void_p, P_int32, P_double]),
     "lulesh_grad_run": (c_int, [c_void_p, c_int64, c_int64, P_double, P_double, P_double,
                                 P_double]),

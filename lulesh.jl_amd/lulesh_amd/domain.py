@@ -151,6 +151,17 @@ class Domain:
     def nodal_depth(self):
         return int(self._lib.lulesh_nodal_depth(self._ctx))
 
+    def set_nodal_tiles(self, lo=0, hi=0):
+        """Allow k_nodal tiles etx in [lo, hi] (0, 0: the default 16..32);
+        results do not depend on it."""
+        _check(self._lib, self._ctx, self._lib.lulesh_set_nodal_tiles(self._ctx, int(lo), int(hi)))
+
+    def nodal_tile(self):
+        """This instance's k_nodal tile: (etx, ety, tiles along x, tiles along y)."""
+        out = (ctypes.c_int32 * 4)()
+        _check(self._lib, self._ctx, self._lib.lulesh_nodal_tile(self._ctx, self._inst, out))
+        return tuple(int(v) for v in out)
+
     # -- field access by reference name --
     def _len(self, name):
         if name in ELEM_FIELDS:
@@ -431,6 +442,9 @@ class Batch:
 
     def set_nodal_depth(self, depth):
         _check(self._lib, self._ctx, self._lib.lulesh_set_nodal_depth(self._ctx, int(depth)))
+
+    def set_nodal_tiles(self, lo=0, hi=0):
+        _check(self._lib, self._ctx, self._lib.lulesh_set_nodal_tiles(self._ctx, int(lo), int(hi)))
 
     def set_timing(self, on=True):
         _check(self._lib, self._ctx, self._lib.lulesh_set_timing(self._ctx, 1 if on else 0))

@@ -52,6 +52,8 @@ def test_argument_validation_before_device_use():
     assert not ctx.value
     assert lib.lulesh_math_check(7, None, None, None, 1, 0) == 100
     assert lib.lulesh_destroy(None) == 100
+    assert lib.lulesh_set_nodal_tiles(None, 16, 32) == 100
+    assert lib.lulesh_nodal_tile(None, 0, None) == 100
     assert lib.lulesh_kernel_name(0).decode() == "k_nodal"
 
 

@@ -102,6 +102,9 @@ def test_autotune_leaves_state_unchanged_and_results_identical():
     before = {(i, f): b.instance(i).get(f) for i in range(len(NX)) for f in FIELDS}
     depth, ms = b.autotune()
     assert depth >= 1 and ms > 0  # instance 0's depth: the model's or a fixed one
+    for i in range(len(NX)):  # a tile of one of the tuned width ranges
+        etx, ety, tx, ty = b.instance(i).nodal_tile()
+        assert 8 <= etx <= 32 and ety == 256 // etx
     for (i, f), a in before.items():
         assert np.array_equal(bits(a), bits(b.instance(i).get(f))), (i, f)
     b.run(60)
@@ -112,6 +115,36 @@ def test_autotune_leaves_state_unchanged_and_results_identical():
         same(b.instance(i), ref.instance(i), f"depth-independence {i}")
     b.close()
     ref.close()
+
+
+@pytest.mark.parametrize("lo,hi", [(16, 16), (12, 21), (8, 32), (24, 32), (2, 128)])
+def test_tile_widths_do_not_change_results(lo, hi):
+    """k_nodal with other tile shapes (down to 2-wide tiles and up to 128
+    wide) ends bit-identical to the default shapes: every node sums its
+    corners in the same order whatever the tiling."""
+    print(f"[tiles] widths {lo}..{hi}", flush=True)
+    b = Batch(NX, SCALE, HG)
+    b.set_nodal_tiles(lo, hi)
+    for i in range(len(NX)):
+        etx, ety, tx, ty = b.instance(i).nodal_tile()
+        assert lo <= etx <= hi and ety == 256 // etx and tx >= 1 and ty >= 1
+    b.run(40)
+    ref = Batch(NX, SCALE, HG)
+    ref.run(40)
+    for i in range(len(NX)):
+        same(b.instance(i), ref.instance(i), f"tiles {lo}..{hi} instance {i}")
+    b.close()
+    ref.close()
+
+
+def test_tile_width_arguments():
+    b = Batch([8])
+    for lo, hi in ((1, 16), (20, 16), (16, 129)):
+        with pytest.raises(Exception):
+            b.set_nodal_tiles(lo, hi)
+    b.set_nodal_tiles(0, 0)
+    assert b.instance(0).nodal_tile()[0] >= 16
+    b.close()
 
 
 def test_error_stops_only_its_instance():

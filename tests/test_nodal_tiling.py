@@ -70,10 +70,18 @@ def test_every_node_owned_once_and_reads_stay_in_tile(etx):
         check_plane(s, etx, ety)
 
 
+@pytest.mark.parametrize("etx", [2, 3, 5, 8, 12, 13, 64, 128])
+def test_other_tile_widths(etx):
+    """widths the tile-range setting and the autotuner may allow"""
+    ety = BLOCK // etx
+    for s in list(range(1, 42)) + [63]:
+        check_plane(s, etx, ety)
+
+
 def test_span_counts_cover_the_axis():
     """the tiles of nodal_span cover s+1 node columns exactly, each owning
     between 1 and a columns (the kernel's lanes per row)"""
-    for a in range(8, 33):
+    for a in list(range(2, 33)) + [64, 128]:
         for s in range(1, 900):
             n = span(s, a)
             i0, oi, iend = tile_nodes(n - 1, a, n, s)
