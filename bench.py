@@ -30,17 +30,35 @@ NODAL_FLOP_PER_ZONE = 1748
 
 
 def alg_bytes(E, N):
-    """Algorithmic HBM bytes per launch of each kernel (unique footprint:
-    every array element a kernel needs counted once, 8 B per double, 4 B per
-    elemBC mask; nodal gathers count each node once)."""
+    """Algorithmic HBM bytes per launch: per element (E) and node (N) of the
+    work the kernel does, the footprint of the reference algorithm for those
+    rows -- each array element a phase needs counted once, 8 B per double,
+    4 B per index or mask, the per-corner force round trip of the
+    atomic-free corner gather at 3 doubles per corner written and read,
+    dead stores excluded.  k_nodal does the force phase (rows 3-6 of the
+    path: nodelist 32E, positions/velocities 48N, p q v volo ss elemMass
+    48E, corner forces written 192E) and the node update (row 7: corner list
+    32E, corner forces read 192E, start/count 8N, nodalMass 8N, positions/
+    velocities read 48N and written 48N); k_kin rows 8-10; k_qeos rows
+    11-14; k_dt reads the k_qeos block minima."""
     return {
-        # x,y,z,xd,yd,zd read (48N) + p,q,v,volo,ss,elemMass (48E) + nodalMass (8N)
-        # + x..zd written (48N); corner forces stay in LDS
+        "k_nodal": (320 * E + 48 * N) + (224 * E + 112 * N),
+        "k_kin": 128 * E + 48 * N,
+        "k_qeos": 196 * E,
+        "k_dt": 16 * ((E + 255) // 256),
+    }
+
+
+def footprint_bytes(E, N):
+    """Bytes each launch of this build's kernels must move at least: the
+    corner forces never leave LDS and connectivity is computed, not read.
+    k_nodal: positions/velocities read 48N and written 48N, nodalMass 8N,
+    p q v volo ss elemMass 48E; k_kin: positions/velocities 48N, volo v 16E,
+    ten outputs 80E; k_qeos: elemBC 4, delv x3 24, delx x3 24, ten inputs
+    80, five outputs 40 (per element)."""
+    return {
         "k_nodal": 48 * E + 104 * N,
-        # x..zd (48N) + volo, v (16E) + vnew delv arealg vdov delx*3 delv*3 written (80E)
         "k_kin": 96 * E + 48 * N,
-        # elemBC 4 + delv*3 24 + delx*3 24 + vdov elemMass volo vnew v delv e p q arealg 80
-        # read, e p q ss v 40 written
         "k_qeos": 172 * E,
         "k_dt": 16 * ((E + 255) // 256),
     }
@@ -199,14 +217,17 @@ def main():
     value = E * ncyc / wall
 
     ab = alg_bytes(E, N)
+    fb = footprint_bytes(E, N)
     kernels = {}
     for name, (ms, cnt) in kt.items():
         if cnt == 0:
             continue
         avg = ms / cnt
         gbs = ab[name] / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+        fgbs = fb[name] / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
         kernels[name] = {"avg_ms": round(avg, 5), "launches": cnt,
-                         "alg_bytes": ab[name], "GB_s": round(gbs, 1)}
+                         "alg_bytes": ab[name], "GB_s": round(gbs, 1),
+                         "footprint_bytes": fb[name], "footprint_GB_s": round(fgbs, 1)}
     dominant = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     dk = kernels[dominant]
     traffic = None
@@ -217,9 +238,14 @@ def main():
                 traffic = json.load(fh).get(dominant)
         except Exception:
             traffic = None
+    # achieved: the algorithmic bytes of the work (the reference algorithm's
+    # footprint, corner round trip included) over the launch time -- the
+    # rate a kernel streaming those bytes would need; traffic: the bytes
+    # this kernel moves (copy-calibrated FETCH_SIZE + WRITE_SIZE), far below
+    # them for k_nodal, whose corner forces stay in LDS
     roofline = {"bound": "hbm", "achieved": dk["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(dk["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": dominant}
+                "kernel": dominant, "footprint_GB_s": dk["footprint_GB_s"]}
 
     # k_nodal is bound by FP64 issue, not HBM: the same kernel against the
     # FP64 peak (its arithmetic is unfused mul/add for bitwise parity, so
@@ -275,6 +301,7 @@ def main():
         "adjoint": adjoint,
         "batched": batched,
         "alg_bytes_per_cycle": sum(ab.values()),
+        "footprint_bytes_per_cycle": sum(fb.values()),
     }
     print(json.dumps(out))
 
