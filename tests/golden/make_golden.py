@@ -1,7 +1,12 @@
 """Writes tests/golden/oracle_runs.json: results of the CPU oracle on the
 parity configurations (verify numbers, cycle count, final time and SHA-256
 of the final energy and x fields).  Run from the repo root after
-`make -C oracle`:  python tests/golden/make_golden.py"""
+`make -C oracle`:  python tests/golden/make_golden.py
+
+With --large it writes tests/golden/oracle_runs_large.json instead: the
+full-size workloads (the s=192 headline's first 500 cycles, s=96 runs),
+SHA-256 of all eleven state fields; minutes of CPU time on 8 threads, so
+only the GPU tests read them back."""
 import hashlib
 import json
 import os
@@ -21,6 +26,14 @@ CASES = [  # (name, nx, quirks, num_iters)
 ]
 
 
+LARGE_CASES = [
+    ("s192_first_500", 192, 7, 500),
+    ("s96_first_2000", 96, 7, 2000),
+    ("s96_quirk_free_to_stoptime", 96, 0, 10 ** 9),
+]
+STATE = ("e", "p", "q", "v", "ss", "x", "y", "z", "xd", "yd", "zd")
+
+
 def digest(a):
     return hashlib.sha256(a.tobytes()).hexdigest()
 
@@ -36,14 +49,30 @@ def run_case(nx, quirks, num_iters, nthreads=8):
             "sha256_e": digest(od.field("e")), "sha256_x": digest(od.field("x"))}
 
 
+def run_large(nx, quirks, num_iters, nthreads):
+    od = O.OracleDomain(nx, quirks, nthreads)
+    rc, _ = od.run(num_iters)
+    v = od.verify()
+    r = {"rc": rc, "cycle": od.get("cycle"), "time": od.get("time").hex(),
+         "deltatime": od.get("deltatime").hex(),
+         "dtcourant": od.get("dtcourant").hex(), "dthydro": od.get("dthydro").hex(),
+         "verify": [float(x).hex() for x in v],
+         "verify_printed": ["%.6e" % x for x in v]}
+    for f in STATE:
+        r["sha256_" + f] = digest(od.field(f))
+    return r
+
+
 def main():
+    large = "--large" in sys.argv
     out = {}
-    for name, nx, q, it in CASES:
-        r = run_case(nx, q, it)
+    for name, nx, q, it in (LARGE_CASES if large else CASES):
+        r = run_large(nx, q, it, os.cpu_count() or 8) if large else run_case(nx, q, it)
         r.update({"nx": nx, "quirks": q, "num_iters": it})
         out[name] = r
-        print(name, r["cycle"], r["verify_printed"])
-    with open(os.path.join(HERE, "oracle_runs.json"), "w") as f:
+        print(name, r["cycle"], r["verify_printed"], flush=True)
+    fn = "oracle_runs_large.json" if large else "oracle_runs.json"
+    with open(os.path.join(HERE, fn), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
 
 

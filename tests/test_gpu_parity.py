@@ -96,9 +96,10 @@ def test_leapfrog_cycles_bitwise(nx, cycles):
 
 @pytest.mark.parametrize("nx,cycles", [(16, 40), (48, 20), (63, 12), (96, 8)])
 def test_nodal_tile_shapes_bitwise(nx, cycles):
-    """Sizes whose k_nodal tile is not 16 x 16 elements (14 x 18 at s=16 and
-    48, 15 x 17 at 63, 12 x 21 at 96) and whose z-chunks are balanced: the
-    driver loop on the device ends bit-identical to the oracle."""
+    """Sizes with every kind of k_nodal tiling (s=16: 17 x 15 lanes, one
+    tile across x and two along y; s=48: 17 x 15, 3 x 4 tiles; s=63: 32 x 8,
+    2 x 9; s=96: 17 x 15, 6 x 7) and balanced z-chunks: the driver loop on
+    the device ends bit-identical to the oracle."""
     print(f"[tiles] nx={nx} cycles={cycles}", flush=True)
     dom, od = make(nx, nthreads=8)
     assert run(dom, cycles) == cycles
@@ -171,6 +172,29 @@ def test_nx192_first_cycles_bitwise():
     assert_same(dom, od, ("e", "p", "q", "v", "ss", "x", "xd", "y", "z"), "nx=192")
     assert_scalars(dom, od, "nx=192")
     dom.close()
+
+
+def test_contexts_are_independent():
+    """Two contexts of different sizes advanced alternately (each on its own
+    stream) end exactly as each does alone: no shared state between
+    contexts."""
+    print("[contexts] nx=10 and nx=14 interleaved", flush=True)
+    a = Domain(LuleshProblem(10, True, 10, 11, 1, 1))
+    b = Domain(LuleshProblem(10, True, 14, 11, 1, 1), quirks=0)
+    for k in range(30):
+        timeIncrement(a)
+        lagrangeLeapFrog(a)
+        if k % 3 == 0:
+            assert run(b, b.cycle + 2) == 2
+    lone_a = Domain(LuleshProblem(10, True, 10, 11, 1, 1))
+    lone_b = Domain(LuleshProblem(10, True, 14, 11, 1, 1), quirks=0)
+    assert run(lone_a, 30) == 30 and run(lone_b, b.cycle) == b.cycle
+    for x, y in ((a, lone_a), (b, lone_b)):
+        for n in STATE_E + STATE_N:
+            assert np.array_equal(bits(x.get(n)), bits(y.get(n))), n
+        assert x.time == y.time and x.cycle == y.cycle
+    for d in (a, b, lone_a, lone_b):
+        d.close()
 
 
 def test_deterministic_reruns_nx96():
