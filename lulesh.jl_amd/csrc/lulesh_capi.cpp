@@ -823,12 +823,15 @@ int adjoint_cycle(lulesh_ctx *c, const AdjArgs &aa, bool last) {
 
 // The gradient run's forward pass: `ncycles` cycles of the driver loop from
 // the current state with timeIncrement! recording each dt; the start state
-// of cycle k is saved into save(k) when that is not null.  Leaves the loss
-// (sum_e e * elemMass, summed on the host in element order), the end state
-// in final_state/final_sc and the dt schedule in B.dt (and dt_out).
+// of cycle k is saved into save(k) when that is not null, and from cycle
+// `slot_from` on each cycle's start state and temporaries go into the
+// segment slots as the reverse sweep's replay would store them (so the
+// last segment needs no replay).  Leaves the loss (sum_e e * elemMass,
+// summed on the host in element order), the end state in
+// final_state/final_sc and the dt schedule in B.dt (and dt_out).
 template <typename Save>
-int grad_forward(lulesh_ctx *c, int64_t ncycles, long long cycle0, Save save, double *loss,
-                 double *dt_out) {
+int grad_forward(lulesh_ctx *c, int64_t ncycles, long long cycle0, Save save, int64_t slot_from,
+                 double *loss, double *dt_out) {
   AdjBuffers &B = c->adj;
   const int64_t E = c->inst[0].E;
   const Batch a = make_args(c, 0, 0);
@@ -839,10 +842,18 @@ int grad_forward(lulesh_ctx *c, int64_t ncycles, long long cycle0, Save save, do
   for (int64_t k = 0; k < ncycles && !rc; ++k) {
     double *buf = save(k);
     if (buf) rc = state_copy(c, buf, 1);
+    if (!rc && k >= slot_from) {
+      const int64_t j = k - slot_from;
+      rc = slot_step(c, j ? slot_temps(c, seg_slot(c, j - 1)) : nullptr, seg_slot(c, j), nullptr, 0);
+    }
     if (!rc) rc = launch(c, KID_TIME_INCREMENT, ta);
     const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
     for (int id : ids)
       if (!rc) rc = launch(c, id, a);
+  }
+  if (!rc && slot_from < ncycles) {
+    const int64_t L = ncycles - slot_from;
+    rc = slot_step(c, slot_temps(c, seg_slot(c, L - 1)), seg_slot(c, L), nullptr, 0);
   }
   if (!rc) rc = finish(c, true);
   if (rc) return rc;
@@ -898,33 +909,36 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
   if (!rc) rc = finish(c, false);
   if (rc) return rc;
   const long long cycle0 = c->hsc[0].cycle;
-  // 1. forward pass: checkpoints every `every` cycles, dt schedule recorded
+  // 1. forward pass: checkpoints every `every` cycles, dt schedule recorded,
+  // the last segment's slots filled on the way
+  const int64_t nseg = (ncycles + every - 1) / every;
   rc = grad_forward(
       c, ncycles, cycle0,
       [&](int64_t k) -> double * { return k % every == 0 ? B.ckpt + (k / every) * state : nullptr; },
-      loss, dt_out);
+      (nseg - 1) * every, loss, dt_out);
   if (rc) return rc;
   // 2. reverse sweep, segment by segment: restore the segment's checkpoint,
   // replay it storing each cycle's start state and temporaries, then run
   // the adjoint kernels of its cycles backwards against the stored slots
   std::vector<double> hdt(ncycles);
   HIP_TRY(c, hipMemcpy(hdt.data(), B.dt, sizeof(double) * ncycles, hipMemcpyDeviceToHost));
-  const int64_t nseg = (ncycles + every - 1) / every;
   for (int64_t sg = nseg - 1; sg >= 0 && !rc; --sg) {
     const int64_t k0 = sg * every, k1 = (k0 + every < ncycles) ? k0 + every : ncycles;
     const int64_t L = k1 - k0;
-    rc = state_copy(c, B.ckpt + sg * state, 0);
-    if (!rc) {
-      hipError_t e = launch_set_cycle(c->stream, c->b.sc, cycle0 + k0);
-      if (e != hipSuccess) rc = hip_fail(c, e, "set cycle");
+    if (sg < nseg - 1) {
+      rc = state_copy(c, B.ckpt + sg * state, 0);
+      if (!rc) {
+        hipError_t e = launch_set_cycle(c->stream, c->b.sc, cycle0 + k0);
+        if (e != hipSuccess) rc = hip_fail(c, e, "set cycle");
+      }
+      // slot j <- S_{k0+j} (and the previous cycle's temporaries), dt, cycle
+      for (int64_t j = 0; j < L && !rc; ++j) {
+        rc = slot_step(c, j ? slot_temps(c, seg_slot(c, j - 1)) : nullptr, seg_slot(c, j), B.dt,
+                       cycle0);
+        if (!rc) rc = cycle_body(c, false);
+      }
+      if (!rc) rc = slot_step(c, slot_temps(c, seg_slot(c, L - 1)), seg_slot(c, L), nullptr, 0);
     }
-    // slot j <- S_{k0+j} (and the previous cycle's temporaries), dt, cycle
-    for (int64_t j = 0; j < L && !rc; ++j) {
-      rc = slot_step(c, j ? slot_temps(c, seg_slot(c, j - 1)) : nullptr, seg_slot(c, j), B.dt,
-                     cycle0);
-      if (!rc) rc = cycle_body(c, false);
-    }
-    if (!rc) rc = slot_step(c, slot_temps(c, seg_slot(c, L - 1)), seg_slot(c, L), nullptr, 0);
     for (int64_t j = L - 1; j >= 0 && !rc; --j) {
       const int64_t k = k0 + j;
       rc = adjoint_cycle(
