@@ -71,8 +71,9 @@ def bench_adjoint(nx=48, cycles=200, every=20, snapshots=10):
     energy after `cycles` cycles (hand-written adjoint kernels, checkpoints
     every `every` cycles, including its own forward pass) over the plain
     forward run of the same cycles, both from the initial state.  Beside it
-    the memory-bounded binomial schedule with `snapshots` saved states (same
-    gradient, bit for bit), with the adjoint memory of each scheme."""
+    the memory-bounded binomial schedule with `snapshots` saved states and
+    the store-everything end (one segment: every cycle's slot kept from the
+    forward pass), same gradient bit for bit, with each scheme's memory."""
     import numpy as np
     from lulesh_amd import Domain, LuleshProblem, grad_run, grad_run_revolve, run
     d = Domain(LuleshProblem(cycles, True, nx, 11, 1, 1))
@@ -93,6 +94,13 @@ def bench_adjoint(nx=48, cycles=200, every=20, snapshots=10):
     t0 = time.perf_counter()
     gr = grad_run_revolve(d, cycles, snapshots)
     grad_rv = time.perf_counter() - t0
+    # every cycle's slot kept from the forward pass: no replay at all
+    d.reset()
+    grad_run(d, cycles, cycles)  # first use allocates the slots
+    d.reset()
+    t0 = time.perf_counter()
+    ga = grad_run(d, cycles, cycles)
+    grad_all = time.perf_counter() - t0
     E, N = d.numElem, d.numNode
     d.close()
     state, slot = 8 * (6 * N + 5 * E), 8 * (6 * N + 14 * E)
@@ -105,7 +113,11 @@ def bench_adjoint(nx=48, cycles=200, every=20, snapshots=10):
             "revolve": {"snapshots": snapshots, "gradient_ms": round(grad_rv * 1e3, 3),
                         "ratio": round(grad_rv / fwd, 3), "cycles_run": gr["forward_cycles"],
                         "checkpoint_MB": round(mem_rv / 1e6, 1),
-                        "same_gradient": bool(np.array_equal(gr["grad_e"], g["grad_e"]))}}
+                        "same_gradient": bool(np.array_equal(gr["grad_e"], g["grad_e"]))},
+            "store_all": {"checkpoint_every": cycles, "gradient_ms": round(grad_all * 1e3, 3),
+                          "ratio": round(grad_all / fwd, 3),
+                          "checkpoint_MB": round((state + (cycles + 1) * slot) / 1e6, 1),
+                          "same_gradient": bool(np.array_equal(ga["grad_e"], g["grad_e"]))}}
 
 
 def bench_batched(n=64, seed=0, cap=5000):
