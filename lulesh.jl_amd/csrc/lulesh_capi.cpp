@@ -66,6 +66,7 @@ struct lulesh_ctx {
   std::vector<void *> allocations;
   int tile_lo = 16, tile_hi = 32;  // k_nodal tile widths allowed (nodal_tile)
   int depth_req = 0;               // k_nodal z-depth asked for (0: the model's)
+  bool nodal_ready = false;        // k_nodal's tables match c->inst (false after a failed plan)
   int *ginst = nullptr;            // device table of gblock_host (regrown by plan_nodal)
   int64_t ginst_cap = 0;
   std::vector<int> gblock_host;  // instance of each k_nodal block
@@ -128,6 +129,8 @@ hipEvent_t get_event(lulesh_ctx *c) {
 
 // one launch on the context stream, bracketed by events when timing is on
 int launch(lulesh_ctx *c, int id, const Batch &a) {
+  if (id == KID_NODAL && !c->nodal_ready)
+    return fail(c, LULESH_ERR_HIP, "k_nodal tables unavailable (a device allocation failed)");
   hipEvent_t ea = nullptr, eb = nullptr;
   const bool timed = c->timing && id < LULESH_K_COUNT;
   if (timed) {
@@ -247,6 +250,8 @@ void release(lulesh_ctx *c) {
 }
 
 int build_graph(lulesh_ctx *c) {
+  if (!c->nodal_ready)
+    return fail(c, LULESH_ERR_HIP, "k_nodal tables unavailable (a device allocation failed)");
   if (c->graph && c->graph_keep_aux == c->keep_aux) return 0;
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
@@ -363,8 +368,13 @@ int order_nodal(lulesh_ctx *c, const std::vector<char> *live) {
   } else {
     for (int k = 0; k < nl; ++k) order[k] = lv[k];
   }
+  c->nodal_ready = false;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if ((int64_t)order.size() > c->gorder_cap) {
+    // only a new plan grows the table (a live subset keeps glaunch), and a
+    // new plan has already dropped the graph that holds the old pointer
+    if (c->graph) return fail(c, LULESH_ERR_HIP, "k_nodal order table would move under the graph");
+    c->b.gblock_order = nullptr;
     if (c->gorder) HIP_TRY(c, hipFree(c->gorder));
     c->gorder = nullptr;
     c->gorder_cap = 0;
@@ -376,10 +386,18 @@ int order_nodal(lulesh_ctx *c, const std::vector<char> *live) {
                             hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->order_live = live ? *live : std::vector<char>();
+  c->nodal_ready = true;
   return 0;
 }
 
 int plan_nodal(lulesh_ctx *c, int cz) {
+  // until the new tables are on the device, k_nodal must not run, and the
+  // captured graph (which holds the old ones) is gone
+  c->nodal_ready = false;
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
   for (InstDesc &I : c->inst) {
     const NodalTile t = nodal_tile(I.s, c->tile_lo, c->tile_hi);
     I.etx = t.etx;
@@ -406,6 +424,7 @@ int plan_nodal(lulesh_ctx *c, int cz) {
   c->b.glaunch = 0;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (gb > c->ginst_cap) {
+    c->b.gblock_inst = nullptr;
     if (c->ginst) HIP_TRY(c, hipFree(c->ginst));
     c->ginst = nullptr;
     c->ginst_cap = 0;
@@ -417,13 +436,7 @@ int plan_nodal(lulesh_ctx *c, int cz) {
                             hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, hipMemcpyAsync(c->ginst, c->gblock_host.data(), sizeof(int) * gb,
                             hipMemcpyHostToDevice, c->stream));
-  const int rc = order_nodal(c, nullptr);
-  if (rc) return rc;
-  if (c->graph) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
-  }
-  return 0;
+  return order_nodal(c, nullptr);
 }
 
 int check_inst(lulesh_ctx *c, int32_t inst) {
@@ -696,6 +709,7 @@ int adj_setup_n(lulesh_ctx *c, int64_t ncycles, int64_t nstates, int64_t seg) {
   const int64_t state = 6 * N + 5 * E;
   int rc = 0;
   if (ncycles > B.dt_cap) {
+    B.dt_cap = 0;  // no buffer until the regrow succeeds
     rc = adj_regrow(c, &B.dt, ncycles);
     if (rc) return rc;
     B.dt_cap = ncycles;
@@ -739,11 +753,13 @@ int adj_setup_n(lulesh_ctx *c, int64_t ncycles, int64_t nstates, int64_t seg) {
     B.ready = true;
   }
   if (nstates > B.ckpt_cap) {
+    B.ckpt_cap = 0;
     rc = adj_regrow(c, &B.ckpt, nstates * state);
     if (rc) return rc;
     B.ckpt_cap = nstates;
   }
   if (seg > B.seg_cap) {
+    B.seg_cap = 0;
     rc = adj_regrow(c, &B.seg, (seg + 1) * cycle_slot(c));
     if (rc) return rc;
     B.seg_cap = seg;
