@@ -160,6 +160,26 @@ def test_quirks_off_nx30_full_run_matches_lulesh2():
     dom.close()
 
 
+def test_every_size_25_to_100_bitwise():
+    """Every domain size from 25 to 100 -- every k_nodal tiling the default
+    tile rule produces in that range -- through 20 cycles of the device
+    driver loop, reference quirks on odd sizes and off on even ones: all
+    state fields and dt bit-identical to the oracle."""
+    nthr = min(16, os.cpu_count() or 1)
+    for nx in range(25, 101):
+        quirks = O.QUIRKS_REFERENCE if nx % 2 else 0
+        print(f"[sizes] nx={nx} quirks={quirks}", flush=True)
+        dom, od = make(nx, quirks=quirks, nthreads=nthr)
+        assert run(dom, 20) == 20
+        rc, _ = od.run(20)
+        assert rc == 0
+        assert_same(dom, od, ("e", "p", "q", "v", "ss", "arealg", "vdov", "x", "y", "z", "xd", "yd",
+                              "zd"), f"nx={nx}")
+        assert dom.time == od.get("time") and dom.deltatime == od.get("deltatime"), nx
+        dom.close()
+        od.close()
+
+
 def test_nx192_first_cycles_bitwise():
     """The headline shape itself (s=192): three cycles, GPU vs oracle."""
     print("[nx192] 3 cycles", flush=True)
