@@ -167,3 +167,50 @@ def test_gradient_quirk_free_matches_finite_differences():
     fd = (loss(lambda x: x.set("hgcoef", hg * (1 + 1e-6))) -
           loss(lambda x: x.set("hgcoef", hg * (1 - 1e-6)))) / (2e-6 * hg)
     assert g["grad_hgcoef"] == pytest.approx(fd, rel=1e-4, abs=1e-9 * abs(g["loss"]))
+
+
+def test_c4_configuration_gradient():
+    """The adjoint benchmark's own configuration: s=48, 200 cycles from the
+    Sedov initial state, checkpoints every 20, loss = total internal energy.
+    d loss / d e[1] and d loss / d {hgcoef, qlc_monoq, qqc_monoq} against
+    central finite differences with a 1e-6 relative step; the 15 other
+    elements drawn by numpy's PCG64 default_rng(0) start at e = 0, inside
+    the EOS energy cut, where the adjoint is exactly 0 and a step below the
+    cut (5e-8 < e_cut = 1e-7) leaves the loss unchanged."""
+    nx, ncyc = 48, 200
+    print("[adjoint] C4: s=48, 200 cycles, checkpoint 20", flush=True)
+    d = fresh(nx)
+    e1 = float(d.e[0])
+    g = grad_run(d, ncyc, 20)
+    d.close()
+    dt = g["dt"]
+
+    def loss(mod):
+        x = fresh(nx)
+        mod(x)
+        replay(x, dt)
+        L = total_internal_energy(x)
+        x.close()
+        return L
+
+    def bump(i, step):
+        def f(x):
+            e = x.e
+            e[i] += step
+            x.e = e
+        return f
+
+    h = 1e-6 * e1
+    fd = (loss(bump(0, h)) - loss(bump(0, -h))) / (2 * h)
+    assert g["grad_e"][0] == pytest.approx(fd, rel=1e-4), (g["grad_e"][0], fd)
+    for name, key in (("hgcoef", "grad_hgcoef"), ("qlc_monoq", "grad_qlc_monoq"),
+                      ("qqc_monoq", "grad_qqc_monoq")):
+        p0 = fresh(nx).get(name)
+        hp = 1e-6 * p0
+        fd = (loss(lambda x: x.set(name, p0 + hp)) - loss(lambda x: x.set(name, p0 - hp))) / (2 * hp)
+        assert g[key] == pytest.approx(fd, rel=1e-4, abs=1e-9 * abs(g["loss"])), (name, g[key], fd)
+    rng = np.random.default_rng(0)
+    others = [int(v) for v in rng.choice(np.arange(1, nx ** 3), 15, replace=False)]
+    for i in others:
+        assert g["grad_e"][i] == 0.0, i
+        assert loss(bump(i, 5e-8)) == loss(bump(i, -5e-8)), i
