@@ -449,6 +449,8 @@ class Batch:
     def set_timing(self, on=True):
         _check(self._lib, self._ctx, self._lib.lulesh_set_timing(self._ctx, 1 if on else 0))
 
+    kernel_times = Domain.kernel_times
+
     def keep_aux(self, on=True):
         _check(self._lib, self._ctx, self._lib.lulesh_set_keep_aux(self._ctx, 1 if on else 0))
 

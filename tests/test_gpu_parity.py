@@ -230,6 +230,10 @@ def test_deterministic_reruns_nx96():
 
 
 # ---- errors: each reference error() message, forced from the host ----
+# Three messages cannot be reached: "Mid Volume Error" re-tests detJ right
+# after integrateStressForElems raised "Early Volume Error" on it
+# (src/domain.jl:1103-1105, 1754-1761); ":2157" and ":2908" re-test vnew
+# after calcKinematicsForElems raised on it (2118, 2168, 2920-2927).
 
 def _err_case(prep_gpu, prep_oracle):
     dom, od = make(8)
@@ -289,6 +293,33 @@ def test_error_inverted_element():
 
     exc, rc = _err_case(lambda d: setattr(d, "x", flip(d.x)), lambda od: od.set_field("x", flip(od.field("x"))))
     assert str(exc) == "Early Volume Error" and exc.code == rc == 1
+
+
+def _set_entries(name, pairs):
+    """(prep_gpu, prep_oracle) that overwrite entries of one field"""
+    def change(a):
+        a = a.copy()
+        for i, val in pairs:
+            a[i] = val(a[i])
+        return a
+
+    return (lambda d: d.set(name, change(d.get(name))),
+            lambda od: od.set_field(name, change(od.field(name))))
+
+
+def test_error_kinematics_volume():
+    """A node fast enough to cross its neighbour within the step: the old
+    positions pass every nodal-phase check, the new volume of element 0 is
+    negative (src/domain.jl:2116-2119)"""
+    exc, rc = _err_case(*_set_entries("xd", [(1, lambda a: -5.0e4)]))
+    assert str(exc) == "Volume Error: calcKinematicsForElems" and exc.code == rc == 4
+
+
+def test_error_eos_volume():
+    """eosvmax < 0 clamps every old relative volume to it
+    (src/domain.jl:2892-2906)"""
+    exc, rc = _err_case(lambda d: setattr(d, "eosvmax", -1.0), lambda od: od.set("eosvmax", -1.0))
+    assert str(exc) == "Volume Error :2887" and exc.code == rc == 8
 
 
 # ---- device arithmetic ----
