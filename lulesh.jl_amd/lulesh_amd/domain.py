@@ -470,27 +470,42 @@ def c5_instances(n=64, seed=0):
     return nx, scale, hg, nr
 
 
+def _julia_num(fmt, x):
+    """x formatted as Julia's @printf writes it: non-finite values as Inf,
+    -Inf and NaN, the rest as C's printf."""
+    if x != x:
+        return "NaN"
+    if x in (float("inf"), float("-inf")):
+        return "Inf" if x > 0 else "-Inf"
+    return fmt % x
+
+
 def verifyAndWriteFinalOutput(elapsed_time, domain, nx, numRanks=1, file=None):
     """Prints the reference's final report (src/utils.jl:124-164)."""
     import sys
     f = file or sys.stdout
     cycle = domain.cycle
     e1, maxabs, totabs, maxrel = verify(domain)
-    grind1 = ((elapsed_time * 1e6) / cycle) / (nx * nx * nx) if cycle else float("nan")
-    grind2 = ((elapsed_time * 1e6) / cycle) / (nx * nx * nx * numRanks) if cycle else float("nan")
+    # IEEE division, as Julia's (a run of 0 cycles gives Inf or NaN, not an exception)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = np.float64(elapsed_time * 1e6) / np.float64(cycle)
+        grind1 = float(t / np.float64(nx * nx * nx))
+        grind2 = float(t / np.float64(nx * nx * nx * numRanks))
+        fom = float(np.float64(1000.0) / np.float64(grind2))
+    jl = _julia_num
     print("Run completed:", file=f)
     print("   Problem size        =  %d" % nx, file=f)
     print("   MPI tasks           =  %d" % numRanks, file=f)
     print("   Iteration count     =  %d" % cycle, file=f)
-    print("   Final Origin Energy =  %.6e" % e1, file=f)
+    print("   Final Origin Energy =  %s" % jl("%.6e", e1), file=f)
     print("   Testing Plane 0 of Energy Array on rank 0:", file=f)
-    print("        MaxAbsDiff   = %.6e" % maxabs, file=f)
-    print("        TotalAbsDiff = %.6e" % totabs, file=f)
-    print("        MaxRelDiff   = %.6e" % maxrel, file=f)
+    print("        MaxAbsDiff   = %s" % jl("%.6e", maxabs), file=f)
+    print("        TotalAbsDiff = %s" % jl("%.6e", totabs), file=f)
+    print("        MaxRelDiff   = %s" % jl("%.6e", maxrel), file=f)
     print("\nElapsed time         = %.2f (s)" % elapsed_time, file=f)
-    print("Grind time (us/z/c)  =  %.8f  (per dom)  (%.8f overall)" % (grind1, elapsed_time),
-          file=f)
-    print("FOM                  = %.8f (z/s)\n" % (1000 / grind2), file=f)
+    print("Grind time (us/z/c)  =  %s  (per dom)  (%s overall)"
+          % (jl("%.8f", grind1), jl("%.8f", elapsed_time)), file=f)
+    print("FOM                  = %s (z/s)\n" % jl("%.8f", fom), file=f)
     return {"cycle": cycle, "e1": e1, "MaxAbsDiff": maxabs, "TotalAbsDiff": totabs,
             "MaxRelDiff": maxrel, "elapsed": elapsed_time}
 
