@@ -48,6 +48,9 @@ extern "C" {
 #define LULESH_ERR_INVALID_ARG 100  /* bad argument (size, name, pointer)  */
 #define LULESH_ERR_HIP 101          /* a HIP runtime call failed           */
 #define LULESH_ERR_UNSUPPORTED 102  /* valid request this build does not do */
+#define LULESH_ERR_HOST_MEMORY 103  /* a host allocation failed (no C++ exception
+                                       ever leaves an entry point); the
+                                       context's state is then unspecified */
 
 /* ---- arithmetic quirks of the reference --------------------------------
  * On by default so results match the reference; a caller may clear them to
@@ -128,6 +131,9 @@ int lulesh_verify(lulesh_ctx *ctx, double out[4]);
  * instances with one launch per kernel and per-instance dt; an instance
  * whose driver loop has ended is skipped, so its state stays exactly what
  * a lone run leaves.  The single-domain entry points address instance 0. */
+/* n <= LULESH_MAX_INSTANCES, each nx[i] in 1..800, and the instances' nodes
+ * (sum of (nx[i]+1)^3) below 2^31; LULESH_ERR_INVALID_ARG otherwise */
+#define LULESH_MAX_INSTANCES (1 << 20)
 int lulesh_batch_create(int32_t n, const int32_t *nx, const double *einit_scale,
                         const double *hgcoef, const int32_t *nr, int32_t quirks, int32_t device,
                         lulesh_ctx **out);

@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <string>
 #include <algorithm>
 #include <vector>
@@ -227,6 +229,7 @@ double *scalar_slot(DevScalars *h, const std::string &s) {
 template <typename T>
 int dev_alloc(lulesh_ctx *c, T **p, int64_t n) {
   void *q = nullptr;
+  c->allocations.reserve(c->allocations.size() + 1);  // the push below cannot throw
   hipError_t e = hipMalloc(&q, (size_t)(n > 0 ? n : 1) * sizeof(T));
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
   c->allocations.push_back(q);
@@ -450,7 +453,8 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
            int32_t device, lulesh_ctx **out) {
   if (!out) return LULESH_ERR_INVALID_ARG;
   *out = nullptr;
-  if (n < 1 || !nx || quirks < 0 || quirks > LULESH_QUIRKS_REFERENCE) return LULESH_ERR_INVALID_ARG;
+  if (n < 1 || n > LULESH_MAX_INSTANCES || !nx || quirks < 0 || quirks > LULESH_QUIRKS_REFERENCE)
+    return LULESH_ERR_INVALID_ARG;
   for (int i = 0; i < n; ++i) {
     // kernels address an instance's node and element arrays with 32-bit byte
     // offsets: (nx+1)^3 * 8 < 2^32 holds up to nx = 811 (s = 800 already
@@ -458,7 +462,23 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
     if (nx[i] < 1 || nx[i] > 800) return LULESH_ERR_INVALID_ARG;
     if (einit_scale && !(einit_scale[i] > 0.0)) return LULESH_ERR_INVALID_ARG;
   }
+  // all instances' nodes together below 2^31: block counts and the host's
+  // block tables stay int-sized (far beyond what one GPU's memory holds)
+  int64_t nodes = 0;
+  for (int i = 0; i < n; ++i) nodes += (int64_t)(nx[i] + 1) * (nx[i] + 1) * (nx[i] + 1);
+  if (nodes > INT32_MAX) return LULESH_ERR_INVALID_ARG;
   lulesh_ctx *c = new lulesh_ctx();
+  // on every exit but success (an error code or an exception) the partly
+  // built context is freed
+  struct Owner {
+    lulesh_ctx *c;
+    ~Owner() {
+      if (c) {
+        release(c);
+        delete c;
+      }
+    }
+  } owner{c};
   c->n = n;
   c->quirks = quirks;
   c->device = device;
@@ -555,11 +575,8 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
   B.nblock_inst = dnb;
   if (!rc) rc = plan_nodal(c, 0);
   if (!rc) rc = lulesh_reset(c);
-  if (rc) {
-    release(c);
-    delete c;
-    return rc;
-  }
+  if (rc) return rc;
+  owner.c = nullptr;
   *out = c;
   return 0;
 }
@@ -1243,37 +1260,75 @@ int shadow_move(lulesh_ctx *c, const char *field, double *host, int64_t n, bool 
   return 0;
 }
 
+// A C++ exception must not cross the C ABI (it would terminate the host
+// process): every entry point that can allocate on the host turns one into
+// a status code and the context's last_error text.
+int exception_status(lulesh_ctx *c) noexcept {
+  int code = LULESH_ERR_HIP;
+  try {
+    throw;
+  } catch (const std::bad_alloc &) {
+    code = LULESH_ERR_HOST_MEMORY;
+    if (c) try { c->last_error = "host memory allocation failed"; } catch (...) {}
+  } catch (const std::exception &e) {
+    if (c) try { c->last_error = std::string("C++ exception: ") + e.what(); } catch (...) {}
+  } catch (...) {
+    if (c) try { c->last_error = "unknown C++ exception"; } catch (...) {}
+  }
+  return code;
+}
+
 }  // namespace
 
 extern "C" {
 
 int lulesh_vjp_cycle(lulesh_ctx *c, double *grad_params) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  return vjp_cycle(c, grad_params);
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    return vjp_cycle(c, grad_params);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_shadow_upload(lulesh_ctx *c, const char *field, const double *src, int64_t n) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  return shadow_move(c, field, const_cast<double *>(src), n, true);
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    return shadow_move(c, field, const_cast<double *>(src), n, true);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_shadow_download(lulesh_ctx *c, const char *field, double *dst, int64_t n) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  return shadow_move(c, field, dst, n, false);
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    return shadow_move(c, field, dst, n, false);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_grad_run(lulesh_ctx *c, int64_t ncycles, int64_t checkpoint_every, double *grad_e,
                     double *grad_params, double *loss, double *dt_schedule) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  return grad_run(c, ncycles, checkpoint_every, grad_e, grad_params, loss, dt_schedule);
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    return grad_run(c, ncycles, checkpoint_every, grad_e, grad_params, loss, dt_schedule);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_grad_run_revolve(lulesh_ctx *c, int64_t ncycles, int64_t snapshots, double *grad_e,
                             double *grad_params, double *loss, double *dt_schedule,
                             int64_t *forward_cycles) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  return grad_run_revolve(c, ncycles, snapshots, grad_e, grad_params, loss, dt_schedule,
-                          forward_cycles);
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    return grad_run_revolve(c, ncycles, snapshots, grad_e, grad_params, loss, dt_schedule,
+                            forward_cycles);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int64_t lulesh_revolve_cycles(int64_t ncycles, int64_t snapshots) {
@@ -1283,24 +1338,28 @@ int64_t lulesh_revolve_cycles(int64_t ncycles, int64_t snapshots) {
 }
 
 int lulesh_replay(lulesh_ctx *c, const double *dt, int64_t ncycles) {
-  if (!c || !dt || ncycles < 0) return LULESH_ERR_INVALID_ARG;
-  if (c->n != 1) return fail(c, LULESH_ERR_UNSUPPORTED, "replay needs a one-domain context");
-  HIP_TRY(c, hipSetDevice(c->device));
-  int rc = adj_setup(c, ncycles > 0 ? ncycles : 1, 0);
-  if (rc) return rc;
-  HIP_TRY(c, hipMemcpy(c->adj.dt, dt, sizeof(double) * ncycles, hipMemcpyHostToDevice));
-  rc = pull_scalars(c);
-  if (!rc) rc = finish(c, false);
-  if (rc) return rc;
-  const long long cycle0 = c->hsc[0].cycle;
-  rc = clear_err(c);
-  for (int64_t k = 0; k < ncycles && !rc; ++k) {
-    rc = slot_step(c, nullptr, nullptr, c->adj.dt, cycle0);
-    if (!rc) rc = cycle_body(c, true);
+  try {
+    if (!c || !dt || ncycles < 0) return LULESH_ERR_INVALID_ARG;
+    if (c->n != 1) return fail(c, LULESH_ERR_UNSUPPORTED, "replay needs a one-domain context");
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = adj_setup(c, ncycles > 0 ? ncycles : 1, 0);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpy(c->adj.dt, dt, sizeof(double) * ncycles, hipMemcpyHostToDevice));
+    rc = pull_scalars(c);
+    if (!rc) rc = finish(c, false);
+    if (rc) return rc;
+    const long long cycle0 = c->hsc[0].cycle;
+    rc = clear_err(c);
+    for (int64_t k = 0; k < ncycles && !rc; ++k) {
+      rc = slot_step(c, nullptr, nullptr, c->adj.dt, cycle0);
+      if (!rc) rc = cycle_body(c, true);
+    }
+    if (rc) return rc;
+    c->partials_valid = false;
+    return finish(c, true);
+  } catch (...) {
+    return exception_status(c);
   }
-  if (rc) return rc;
-  c->partials_valid = false;
-  return finish(c, true);
 }
 
 const char *lulesh_error_string(int code) {
@@ -1309,38 +1368,55 @@ const char *lulesh_error_string(int code) {
   if (code == LULESH_ERR_INVALID_ARG) return "invalid argument";
   if (code == LULESH_ERR_HIP) return "HIP runtime error";
   if (code == LULESH_ERR_UNSUPPORTED) return "unsupported";
+  if (code == LULESH_ERR_HOST_MEMORY) return "host memory allocation failed";
   return "unknown error";
 }
 
 int lulesh_create(int32_t nx, int32_t nr, int32_t balance, int32_t cost, int32_t quirks,
                   int32_t device, lulesh_ctx **out) {
-  return create(1, &nx, nullptr, nullptr, &nr, &balance, &cost, quirks, device, out);
+  try {
+    return create(1, &nx, nullptr, nullptr, &nr, &balance, &cost, quirks, device, out);
+  } catch (...) {
+    return exception_status(nullptr);
+  }
 }
 
 int lulesh_batch_create(int32_t n, const int32_t *nx, const double *einit_scale,
                         const double *hgcoef, const int32_t *nr, int32_t quirks, int32_t device,
                         lulesh_ctx **out) {
-  return create(n, nx, einit_scale, hgcoef, nr, nullptr, nullptr, quirks, device, out);
+  try {
+    return create(n, nx, einit_scale, hgcoef, nr, nullptr, nullptr, quirks, device, out);
+  } catch (...) {
+    return exception_status(nullptr);
+  }
 }
 
 int32_t lulesh_batch_size(const lulesh_ctx *c) { return c ? c->n : -1; }
 
 int lulesh_destroy(lulesh_ctx *c) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  release(c);
-  delete c;
-  return 0;
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    release(c);
+    delete c;
+    return 0;
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_reset(lulesh_ctx *c) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  hipError_t e = launch_init(c->stream, c->b);
-  if (e != hipSuccess) return hip_fail(c, e, "init kernels");
-  c->partials_valid = false;
-  return finish(c, false);
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipError_t e = launch_init(c->stream, c->b);
+    if (e != hipSuccess) return hip_fail(c, e, "init kernels");
+    c->partials_valid = false;
+    return finish(c, false);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int64_t lulesh_num_elem(const lulesh_ctx *c) { return c ? c->inst[0].E : -1; }
@@ -1354,191 +1430,271 @@ int64_t lulesh_batch_num_node(const lulesh_ctx *c, int32_t inst) {
 const char *lulesh_last_error(const lulesh_ctx *c) { return c ? c->last_error.c_str() : "null context"; }
 
 int lulesh_batch_download(lulesh_ctx *c, int32_t inst, const char *field, double *dst, int64_t n) {
-  int rc = check_inst(c, inst);
-  if (rc) return rc;
-  if (!dst) return LULESH_ERR_INVALID_ARG;
-  DevScalars s;
-  rc = get_inst_scalars(c, inst, &s);
-  if (rc) return rc;
-  FieldRef r = find_field(c, inst, field, s.parity);
-  if (!r.ptr) return fail(c, LULESH_ERR_INVALID_ARG, std::string("unknown field ") + (field ? field : "(null)"));
-  if (n != r.n) return fail(c, LULESH_ERR_INVALID_ARG, "field length mismatch");
-  HIP_TRY(c, hipMemcpyAsync(dst, r.ptr, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  return finish(c, false);
+  try {
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    if (!dst) return LULESH_ERR_INVALID_ARG;
+    DevScalars s;
+    rc = get_inst_scalars(c, inst, &s);
+    if (rc) return rc;
+    FieldRef r = find_field(c, inst, field, s.parity);
+    if (!r.ptr) return fail(c, LULESH_ERR_INVALID_ARG, std::string("unknown field ") + (field ? field : "(null)"));
+    if (n != r.n) return fail(c, LULESH_ERR_INVALID_ARG, "field length mismatch");
+    HIP_TRY(c, hipMemcpyAsync(dst, r.ptr, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return finish(c, false);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_batch_upload(lulesh_ctx *c, int32_t inst, const char *field, const double *src, int64_t n) {
-  int rc = check_inst(c, inst);
-  if (rc) return rc;
-  if (!src) return LULESH_ERR_INVALID_ARG;
-  DevScalars s;
-  rc = get_inst_scalars(c, inst, &s);
-  if (rc) return rc;
-  FieldRef r = find_field(c, inst, field, s.parity);
-  if (!r.ptr) return fail(c, LULESH_ERR_INVALID_ARG, std::string("unknown field ") + (field ? field : "(null)"));
-  if (n != r.n) return fail(c, LULESH_ERR_INVALID_ARG, "field length mismatch");
-  HIP_TRY(c, hipMemcpyAsync(r.ptr, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  c->partials_valid = false;
-  return finish(c, false);
+  try {
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    if (!src) return LULESH_ERR_INVALID_ARG;
+    DevScalars s;
+    rc = get_inst_scalars(c, inst, &s);
+    if (rc) return rc;
+    FieldRef r = find_field(c, inst, field, s.parity);
+    if (!r.ptr) return fail(c, LULESH_ERR_INVALID_ARG, std::string("unknown field ") + (field ? field : "(null)"));
+    if (n != r.n) return fail(c, LULESH_ERR_INVALID_ARG, "field length mismatch");
+    HIP_TRY(c, hipMemcpyAsync(r.ptr, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->partials_valid = false;
+    return finish(c, false);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_download(lulesh_ctx *c, const char *field, double *dst, int64_t n) {
-  return lulesh_batch_download(c, 0, field, dst, n);
+  try {
+    return lulesh_batch_download(c, 0, field, dst, n);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 int lulesh_upload(lulesh_ctx *c, const char *field, const double *src, int64_t n) {
-  return lulesh_batch_upload(c, 0, field, src, n);
+  try {
+    return lulesh_batch_upload(c, 0, field, src, n);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_batch_download_elem_bc(lulesh_ctx *c, int32_t inst, uint32_t *dst, int64_t n) {
-  int rc = check_inst(c, inst);
-  if (rc) return rc;
-  if (!dst || n != c->inst[inst].E) return fail(c, LULESH_ERR_INVALID_ARG, "elemBC length mismatch");
-  HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpyAsync(dst, c->b.f.elemBC + c->inst[inst].e0, (size_t)n * 4,
-                            hipMemcpyDeviceToHost, c->stream));
-  return finish(c, false);
+  try {
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    if (!dst || n != c->inst[inst].E) return fail(c, LULESH_ERR_INVALID_ARG, "elemBC length mismatch");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(dst, c->b.f.elemBC + c->inst[inst].e0, (size_t)n * 4,
+                              hipMemcpyDeviceToHost, c->stream));
+    return finish(c, false);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_batch_upload_elem_bc(lulesh_ctx *c, int32_t inst, const uint32_t *src, int64_t n) {
-  int rc = check_inst(c, inst);
-  if (rc) return rc;
-  if (!src || n != c->inst[inst].E) return fail(c, LULESH_ERR_INVALID_ARG, "elemBC length mismatch");
-  HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpyAsync(c->b.f.elemBC + c->inst[inst].e0, src, (size_t)n * 4,
-                            hipMemcpyHostToDevice, c->stream));
-  return finish(c, false);
+  try {
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    if (!src || n != c->inst[inst].E) return fail(c, LULESH_ERR_INVALID_ARG, "elemBC length mismatch");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(c->b.f.elemBC + c->inst[inst].e0, src, (size_t)n * 4,
+                              hipMemcpyHostToDevice, c->stream));
+    return finish(c, false);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_download_elem_bc(lulesh_ctx *c, uint32_t *dst, int64_t n) {
-  return lulesh_batch_download_elem_bc(c, 0, dst, n);
+  try {
+    return lulesh_batch_download_elem_bc(c, 0, dst, n);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 int lulesh_upload_elem_bc(lulesh_ctx *c, const uint32_t *src, int64_t n) {
-  return lulesh_batch_upload_elem_bc(c, 0, src, n);
+  try {
+    return lulesh_batch_upload_elem_bc(c, 0, src, n);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_batch_get_scalar(lulesh_ctx *c, int32_t inst, const char *name, double *value) {
-  int rc = check_inst(c, inst);
-  if (rc) return rc;
-  if (!name || !value) return LULESH_ERR_INVALID_ARG;
-  DevScalars s;
-  rc = get_inst_scalars(c, inst, &s);
-  if (rc) return rc;
-  const std::string nm(name);
-  if (nm == "cycle") {
-    *value = (double)s.cycle;
+  try {
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    if (!name || !value) return LULESH_ERR_INVALID_ARG;
+    DevScalars s;
+    rc = get_inst_scalars(c, inst, &s);
+    if (rc) return rc;
+    const std::string nm(name);
+    if (nm == "cycle") {
+      *value = (double)s.cycle;
+      return 0;
+    }
+    double *slot = scalar_slot(&s, nm);
+    if (!slot) return fail(c, LULESH_ERR_INVALID_ARG, "unknown scalar " + nm);
+    *value = *slot;
     return 0;
+  } catch (...) {
+    return exception_status(c);
   }
-  double *slot = scalar_slot(&s, nm);
-  if (!slot) return fail(c, LULESH_ERR_INVALID_ARG, "unknown scalar " + nm);
-  *value = *slot;
-  return 0;
 }
 
 int lulesh_batch_set_scalar(lulesh_ctx *c, int32_t inst, const char *name, double value) {
-  int rc = check_inst(c, inst);
-  if (rc) return rc;
-  if (!name) return LULESH_ERR_INVALID_ARG;
-  DevScalars s;
-  rc = get_inst_scalars(c, inst, &s);
-  if (rc) return rc;
-  const std::string nm(name);
-  if (nm == "cycle") {
-    if (!(value >= 0.0) || value != std::floor(value))
-      return fail(c, LULESH_ERR_INVALID_ARG, "cycle must be a non-negative integer");
-    s.cycle = (long long)value;
-  } else {
-    double *slot = scalar_slot(&s, nm);
-    if (!slot) return fail(c, LULESH_ERR_INVALID_ARG, "unknown scalar " + nm);
-    *slot = value;
+  try {
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    if (!name) return LULESH_ERR_INVALID_ARG;
+    DevScalars s;
+    rc = get_inst_scalars(c, inst, &s);
+    if (rc) return rc;
+    const std::string nm(name);
+    if (nm == "cycle") {
+      if (!(value >= 0.0) || value != std::floor(value))
+        return fail(c, LULESH_ERR_INVALID_ARG, "cycle must be a non-negative integer");
+      s.cycle = (long long)value;
+    } else {
+      double *slot = scalar_slot(&s, nm);
+      if (!slot) return fail(c, LULESH_ERR_INVALID_ARG, "unknown scalar " + nm);
+      *slot = value;
+    }
+    c->hsc[inst] = s;
+    HIP_TRY(c, hipMemcpyAsync(c->b.sc + inst, c->hsc + inst, sizeof(DevScalars),
+                              hipMemcpyHostToDevice, c->stream));
+    c->partials_valid = false;
+    return finish(c, false);
+  } catch (...) {
+    return exception_status(c);
   }
-  c->hsc[inst] = s;
-  HIP_TRY(c, hipMemcpyAsync(c->b.sc + inst, c->hsc + inst, sizeof(DevScalars),
-                            hipMemcpyHostToDevice, c->stream));
-  c->partials_valid = false;
-  return finish(c, false);
 }
 
 int lulesh_get_scalar(lulesh_ctx *c, const char *name, double *value) {
-  return lulesh_batch_get_scalar(c, 0, name, value);
+  try {
+    return lulesh_batch_get_scalar(c, 0, name, value);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 int lulesh_set_scalar(lulesh_ctx *c, const char *name, double value) {
-  return lulesh_batch_set_scalar(c, 0, name, value);
+  try {
+    return lulesh_batch_set_scalar(c, 0, name, value);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_set_keep_aux(lulesh_ctx *c, int32_t on) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  c->keep_aux = on ? 1 : 0;
-  return 0;
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    c->keep_aux = on ? 1 : 0;
+    return 0;
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_time_increment(lulesh_ctx *c) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  int rc = launch(c, KID_TIME_INCREMENT, make_args(c, 0, 0));
-  if (rc) return rc;
-  return finish(c, false);
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = launch(c, KID_TIME_INCREMENT, make_args(c, 0, 0));
+    if (rc) return rc;
+    return finish(c, false);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_lagrange_nodal(lulesh_ctx *c) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  const Batch a = make_args(c, 0, 0);
-  int rc = clear_err(c);
-  if (!rc) rc = launch(c, KID_NODAL, a);
-  if (rc) return rc;
-  c->partials_valid = false;
-  return finish(c, true);
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const Batch a = make_args(c, 0, 0);
+    int rc = clear_err(c);
+    if (!rc) rc = launch(c, KID_NODAL, a);
+    if (rc) return rc;
+    c->partials_valid = false;
+    return finish(c, true);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_lagrange_elements(lulesh_ctx *c) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  const Batch a = make_args(c, 0, 0);
-  int rc = clear_err(c);
-  if (!rc) rc = launch(c, KID_KIN, a);
-  if (!rc) rc = launch(c, KID_QEOS, a);
-  if (rc) return rc;
-  rc = finish(c, true);
-  c->partials_valid = (rc == 0);
-  return rc;
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const Batch a = make_args(c, 0, 0);
+    int rc = clear_err(c);
+    if (!rc) rc = launch(c, KID_KIN, a);
+    if (!rc) rc = launch(c, KID_QEOS, a);
+    if (rc) return rc;
+    rc = finish(c, true);
+    c->partials_valid = (rc == 0);
+    return rc;
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_calc_time_constraints(lulesh_ctx *c) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  const Batch a = make_args(c, 0, 0);
-  int rc = 0;
-  if (!c->partials_valid) rc = launch(c, KID_DTPART, a);
-  if (!rc) rc = launch(c, KID_DT, a);
-  if (rc) return rc;
-  c->partials_valid = false;
-  return finish(c, false);
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const Batch a = make_args(c, 0, 0);
+    int rc = 0;
+    if (!c->partials_valid) rc = launch(c, KID_DTPART, a);
+    if (!rc) rc = launch(c, KID_DT, a);
+    if (rc) return rc;
+    c->partials_valid = false;
+    return finish(c, false);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_lagrange_leapfrog(lulesh_ctx *c) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  const Batch a = make_args(c, 0, 0);
-  int rc = clear_err(c);
-  const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
-  for (int id : ids)
-    if (!rc) rc = launch(c, id, a);
-  if (rc) return rc;
-  c->partials_valid = false;
-  return finish(c, true);
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const Batch a = make_args(c, 0, 0);
+    int rc = clear_err(c);
+    const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
+    for (int id : ids)
+      if (!rc) rc = launch(c, id, a);
+    if (rc) return rc;
+    c->partials_valid = false;
+    return finish(c, true);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_run(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  std::vector<int64_t> cyc(c->n, 0);
-  int rc = run_cycles(c, num_iters, cyc.data());
-  if (cycles_run) *cycles_run = cyc[0];
-  return rc;
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    std::vector<int64_t> cyc(c->n, 0);
+    int rc = run_cycles(c, num_iters, cyc.data());
+    if (cycles_run) *cycles_run = cyc[0];
+    return rc;
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_batch_run(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  return run_cycles(c, num_iters, cycles_run);
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    return run_cycles(c, num_iters, cycles_run);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_batch_error(const lulesh_ctx *c, int32_t inst) {
@@ -1548,92 +1704,116 @@ int lulesh_batch_error(const lulesh_ctx *c, int32_t inst) {
 }
 
 int lulesh_batch_verify(lulesh_ctx *c, int32_t inst, double out[4]) {
-  int rc = check_inst(c, inst);
-  if (rc) return rc;
-  if (!out) return LULESH_ERR_INVALID_ARG;
-  const int64_t nx = c->inst[inst].s;
-  std::vector<double> e(nx * nx);
-  HIP_TRY(c, hipSetDevice(c->device));
-  HIP_TRY(c, hipMemcpyAsync(e.data(), c->b.f.e + c->inst[inst].e0, e.size() * sizeof(double),
-                            hipMemcpyDeviceToHost, c->stream));
-  rc = finish(c, false);
-  if (rc) return rc;
-  // src/utils.jl:134-152
-  double MaxAbsDiff = 0.0, MaxRelDiff = 0.0, TotalAbsDiff = 0.0;
-  for (int64_t j = 1; j <= nx; ++j)
-    for (int64_t k = j + 1; k <= nx; ++k) {
-      const double a = e[(j - 1) * nx + k - 1], b = e[(k - 1) * nx + j - 1];
-      const double AbsDiff = std::fabs(a - b);
-      TotalAbsDiff = TotalAbsDiff + AbsDiff;
-      if (MaxAbsDiff < AbsDiff) MaxAbsDiff = AbsDiff;
-      const double RelDiff = AbsDiff / b;
-      if (MaxRelDiff < RelDiff) MaxRelDiff = RelDiff;
-    }
-  out[0] = e[0];
-  out[1] = MaxAbsDiff;
-  out[2] = TotalAbsDiff;
-  out[3] = MaxRelDiff;
-  return 0;
+  try {
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    if (!out) return LULESH_ERR_INVALID_ARG;
+    const int64_t nx = c->inst[inst].s;
+    std::vector<double> e(nx * nx);
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(e.data(), c->b.f.e + c->inst[inst].e0, e.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    rc = finish(c, false);
+    if (rc) return rc;
+    // src/utils.jl:134-152
+    double MaxAbsDiff = 0.0, MaxRelDiff = 0.0, TotalAbsDiff = 0.0;
+    for (int64_t j = 1; j <= nx; ++j)
+      for (int64_t k = j + 1; k <= nx; ++k) {
+        const double a = e[(j - 1) * nx + k - 1], b = e[(k - 1) * nx + j - 1];
+        const double AbsDiff = std::fabs(a - b);
+        TotalAbsDiff = TotalAbsDiff + AbsDiff;
+        if (MaxAbsDiff < AbsDiff) MaxAbsDiff = AbsDiff;
+        const double RelDiff = AbsDiff / b;
+        if (MaxRelDiff < RelDiff) MaxRelDiff = RelDiff;
+      }
+    out[0] = e[0];
+    out[1] = MaxAbsDiff;
+    out[2] = TotalAbsDiff;
+    out[3] = MaxRelDiff;
+    return 0;
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_verify(lulesh_ctx *c, double out[4]) { return lulesh_batch_verify(c, 0, out); }
 
 int lulesh_set_timing(lulesh_ctx *c, int32_t on) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  c->timing = on != 0;
-  for (int i = 0; i < LULESH_K_COUNT; ++i) {
-    c->ktime[i] = 0.0;
-    c->kcount[i] = 0;
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    c->timing = on != 0;
+    for (int i = 0; i < LULESH_K_COUNT; ++i) {
+      c->ktime[i] = 0.0;
+      c->kcount[i] = 0;
+    }
+    return 0;
+  } catch (...) {
+    return exception_status(c);
   }
-  return 0;
 }
 
 int lulesh_kernel_time(lulesh_ctx *c, int32_t id, double *total_ms, int64_t *launches) {
-  if (!c || id < 0 || id >= LULESH_K_COUNT) return LULESH_ERR_INVALID_ARG;
-  if (total_ms) *total_ms = c->ktime[id];
-  if (launches) *launches = c->kcount[id];
-  return 0;
+  try {
+    if (!c || id < 0 || id >= LULESH_K_COUNT) return LULESH_ERR_INVALID_ARG;
+    if (total_ms) *total_ms = c->ktime[id];
+    if (launches) *launches = c->kcount[id];
+    return 0;
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_math_check(int32_t op, const double *a, const double *b, double *out, int64_t n,
                       int32_t device) {
-  if (op < 0 || op > 2 || !a || !out || n < 0 || (op == 2 && !b)) return LULESH_ERR_INVALID_ARG;
-  if (n == 0) return 0;
-  if (hipSetDevice(device) != hipSuccess) return LULESH_ERR_HIP;
-  double *d = nullptr;
-  const size_t bytes = (size_t)n * sizeof(double);
-  if (hipMalloc(&d, 3 * bytes) != hipSuccess) return LULESH_ERR_HIP;
-  int rc = 0;
-  if (hipMemcpy(d, a, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = LULESH_ERR_HIP;
-  if (!rc && op == 2 && hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice) != hipSuccess)
-    rc = LULESH_ERR_HIP;
-  if (!rc && launch_math(nullptr, op, d, d + n, d + 2 * n, n) != hipSuccess) rc = LULESH_ERR_HIP;
-  if (!rc && hipDeviceSynchronize() != hipSuccess) rc = LULESH_ERR_HIP;
-  if (!rc && hipMemcpy(out, d + 2 * n, bytes, hipMemcpyDeviceToHost) != hipSuccess)
-    rc = LULESH_ERR_HIP;
-  (void)hipFree(d);
-  return rc;
+  try {
+    if (op < 0 || op > 2 || !a || !out || n < 0 || (op == 2 && !b)) return LULESH_ERR_INVALID_ARG;
+    if (n == 0) return 0;
+    if (hipSetDevice(device) != hipSuccess) return LULESH_ERR_HIP;
+    double *d = nullptr;
+    const size_t bytes = (size_t)n * sizeof(double);
+    if (hipMalloc(&d, 3 * bytes) != hipSuccess) return LULESH_ERR_HIP;
+    int rc = 0;
+    if (hipMemcpy(d, a, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = LULESH_ERR_HIP;
+    if (!rc && op == 2 && hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice) != hipSuccess)
+      rc = LULESH_ERR_HIP;
+    if (!rc && launch_math(nullptr, op, d, d + n, d + 2 * n, n) != hipSuccess) rc = LULESH_ERR_HIP;
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = LULESH_ERR_HIP;
+    if (!rc && hipMemcpy(out, d + 2 * n, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = LULESH_ERR_HIP;
+    (void)hipFree(d);
+    return rc;
+  } catch (...) {
+    return exception_status(nullptr);
+  }
 }
 
 int lulesh_set_nodal_depth(lulesh_ctx *c, int32_t cz) {
-  if (!c || cz < 0 || cz > 1024) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  c->depth_req = cz;
-  return plan_nodal(c, cz);
+  try {
+    if (!c || cz < 0 || cz > 1024) return LULESH_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->depth_req = cz;
+    return plan_nodal(c, cz);
+  } catch (...) {
+    return exception_status(c);
+  }
 }
 
 int lulesh_set_nodal_tiles(lulesh_ctx *c, int32_t lo, int32_t hi) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  if (lo == 0 && hi == 0) {
-    lo = 16;
-    hi = 32;
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    if (lo == 0 && hi == 0) {
+      lo = 16;
+      hi = 32;
+    }
+    if (lo < 2 || hi < lo || hi > BLOCK / 2)
+      return fail(c, LULESH_ERR_INVALID_ARG, "tile widths must satisfy 2 <= lo <= hi <= 128");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->tile_lo = lo;
+    c->tile_hi = hi;
+    return plan_nodal(c, c->depth_req);
+  } catch (...) {
+    return exception_status(c);
   }
-  if (lo < 2 || hi < lo || hi > BLOCK / 2)
-    return fail(c, LULESH_ERR_INVALID_ARG, "tile widths must satisfy 2 <= lo <= hi <= 128");
-  HIP_TRY(c, hipSetDevice(c->device));
-  c->tile_lo = lo;
-  c->tile_hi = hi;
-  return plan_nodal(c, c->depth_req);
 }
 
 int lulesh_nodal_tile(const lulesh_ctx *c, int32_t inst, int32_t *shape) {
@@ -1649,62 +1829,66 @@ int lulesh_nodal_tile(const lulesh_ctx *c, int32_t inst, int32_t *shape) {
 int32_t lulesh_nodal_depth(const lulesh_ctx *c) { return c ? c->inst[0].cz : -1; }
 
 int lulesh_autotune(lulesh_ctx *c, int32_t *chosen, double *best_ms) {
-  if (!c) return LULESH_ERR_INVALID_ARG;
-  HIP_TRY(c, hipSetDevice(c->device));
-  // k_nodal with its parity flip switched off reads the current node set
-  // and writes only the other one, so it can be timed on the live state
-  // without changing it.  Candidates: tile-width ranges (the default first)
-  // x z-depths (the model's first).
-  const int fam[][2] = {{16, 32}, {16, 16}, {12, 21}, {8, 32}, {24, 32}};
-  const int cand[] = {0, 2, 4, 8, 16, 32};
-  int maxs = 0;
-  for (const InstDesc &I : c->inst) maxs = I.s > maxs ? I.s : maxs;
-  int best_f = -1, best_cz = -1;
-  float best_t = 0.f;
-  hipEvent_t ea = get_event(c), eb = get_event(c);
-  if (!ea || !eb) return fail(c, LULESH_ERR_HIP, "hipEventCreate failed");
-  int rc = 0;
-  for (int fi = 0; fi < (int)(sizeof(fam) / sizeof(fam[0])) && !rc; ++fi) {
-    c->tile_lo = fam[fi][0];
-    c->tile_hi = fam[fi][1];
-    for (int cz : cand) {
-      if (cz > 2 && cz / 2 > maxs + 1) break;
-      rc = plan_nodal(c, cz);
-      if (rc) break;
-      Batch a = make_args(c, 0, 0);
-      a.flip = 0;
-      hipError_t e = hipSuccess;
-      for (int w = 0; w < 2 && e == hipSuccess; ++w) e = launch_kernel(KID_NODAL, c->stream, a);
-      if (e == hipSuccess) e = hipEventRecord(ea, c->stream);
-      for (int r = 0; r < 5 && e == hipSuccess; ++r) e = launch_kernel(KID_NODAL, c->stream, a);
-      if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
-      if (e == hipSuccess) e = hipEventSynchronize(eb);
-      float ms = 0.f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
-      if (e != hipSuccess) {
-        rc = hip_fail(c, e, "autotune");
-        break;
-      }
-      if (best_f < 0 || ms < best_t) {
-        best_f = fi;
-        best_cz = cz;
-        best_t = ms;
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    // k_nodal with its parity flip switched off reads the current node set
+    // and writes only the other one, so it can be timed on the live state
+    // without changing it.  Candidates: tile-width ranges (the default first)
+    // x z-depths (the model's first).
+    const int fam[][2] = {{16, 32}, {16, 16}, {12, 21}, {8, 32}, {24, 32}};
+    const int cand[] = {0, 2, 4, 8, 16, 32};
+    int maxs = 0;
+    for (const InstDesc &I : c->inst) maxs = I.s > maxs ? I.s : maxs;
+    int best_f = -1, best_cz = -1;
+    float best_t = 0.f;
+    hipEvent_t ea = get_event(c), eb = get_event(c);
+    if (!ea || !eb) return fail(c, LULESH_ERR_HIP, "hipEventCreate failed");
+    int rc = 0;
+    for (int fi = 0; fi < (int)(sizeof(fam) / sizeof(fam[0])) && !rc; ++fi) {
+      c->tile_lo = fam[fi][0];
+      c->tile_hi = fam[fi][1];
+      for (int cz : cand) {
+        if (cz > 2 && cz / 2 > maxs + 1) break;
+        rc = plan_nodal(c, cz);
+        if (rc) break;
+        Batch a = make_args(c, 0, 0);
+        a.flip = 0;
+        hipError_t e = hipSuccess;
+        for (int w = 0; w < 2 && e == hipSuccess; ++w) e = launch_kernel(KID_NODAL, c->stream, a);
+        if (e == hipSuccess) e = hipEventRecord(ea, c->stream);
+        for (int r = 0; r < 5 && e == hipSuccess; ++r) e = launch_kernel(KID_NODAL, c->stream, a);
+        if (e == hipSuccess) e = hipEventRecord(eb, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(eb);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
+        if (e != hipSuccess) {
+          rc = hip_fail(c, e, "autotune");
+          break;
+        }
+        if (best_f < 0 || ms < best_t) {
+          best_f = fi;
+          best_cz = cz;
+          best_t = ms;
+        }
       }
     }
+    c->event_pool.push_back(ea);
+    c->event_pool.push_back(eb);
+    if (rc) return rc;
+    c->tile_lo = fam[best_f][0];
+    c->tile_hi = fam[best_f][1];
+    c->depth_req = best_cz;
+    rc = plan_nodal(c, best_cz);
+    if (!rc) rc = clear_err(c);
+    if (!rc) rc = finish(c, false);
+    if (rc) return rc;
+    if (chosen) *chosen = c->inst[0].cz;
+    if (best_ms) *best_ms = best_t / 5.0;
+    return 0;
+  } catch (...) {
+    return exception_status(c);
   }
-  c->event_pool.push_back(ea);
-  c->event_pool.push_back(eb);
-  if (rc) return rc;
-  c->tile_lo = fam[best_f][0];
-  c->tile_hi = fam[best_f][1];
-  c->depth_req = best_cz;
-  rc = plan_nodal(c, best_cz);
-  if (!rc) rc = clear_err(c);
-  if (!rc) rc = finish(c, false);
-  if (rc) return rc;
-  if (chosen) *chosen = c->inst[0].cz;
-  if (best_ms) *best_ms = best_t / 5.0;
-  return 0;
 }
 
 const char *lulesh_kernel_name(int32_t id) {

@@ -55,6 +55,14 @@ def test_argument_validation_before_device_use():
     assert lib.lulesh_set_nodal_tiles(None, 16, 32) == 100
     assert lib.lulesh_nodal_tile(None, 0, None) == 100
     assert lib.lulesh_kernel_name(0).decode() == "k_nodal"
+    # batch limits: instance count, and nodes of all instances below 2^31
+    big = 1 << 20
+    assert lib.lulesh_batch_create(big + 1, (ctypes.c_int32 * 1)(8), None, None, None, 7, 0,
+                                   ctypes.byref(ctx)) == 100
+    nx = (ctypes.c_int32 * 5)(*[800] * 5)  # 5 x 801^3 nodes > 2^31
+    assert lib.lulesh_batch_create(5, nx, None, None, None, 7, 0, ctypes.byref(ctx)) == 100
+    assert not ctx.value
+    assert lib.lulesh_error_string(103).decode() == "host memory allocation failed"
 
 
 def test_missing_library_fails_loudly(monkeypatch):
