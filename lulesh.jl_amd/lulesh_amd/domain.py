@@ -405,6 +405,11 @@ class Batch:
 
     def close(self):
         if getattr(self, "_ctx", None):
+            # the instance views hold the same context: drop it there too, so
+            # a later call through a view fails with an argument error
+            # instead of reaching freed memory
+            for v in getattr(self, "views", ()):
+                object.__setattr__(v, "_ctx", None)
             self._lib.lulesh_destroy(self._ctx)
             self._ctx = None
 

@@ -172,3 +172,25 @@ def test_error_stops_only_its_instance():
         b2.run(5)
     assert "Error" in str(ei.value)
     b.close()
+
+
+def test_closed_batch_and_domain_refuse_calls():
+    """After close() the instance views and the domain raise LuleshError
+    (the library's null-context check) rather than reach freed memory."""
+    from lulesh_amd import Domain, LuleshError, LuleshProblem, lagrangeLeapFrog, timeIncrement
+    print("[close] views after close", flush=True)
+    b = Batch([4, 5])
+    v = b.instance(1)
+    assert v.get("e").size == 125
+    b.close()
+    with pytest.raises(LuleshError):
+        v.get("e")
+    with pytest.raises(LuleshError):
+        timeIncrement(v)
+    b.close()  # a second close is a no-op
+    d = Domain(LuleshProblem(10, True, 4, 11, 1, 1))
+    d.close()
+    with pytest.raises(LuleshError):
+        lagrangeLeapFrog(d)
+    with pytest.raises(LuleshError):
+        d.get("x")
