@@ -345,12 +345,14 @@ def vjp_cycle(domain, shadow=None, grad_params=None):
     None to use the device-resident shadow as it stands) is loaded first and
     the returned dict holds the new shadow; `grad_params` (3 floats: hgcoef,
     qlc_monoq, qqc_monoq) accumulates."""
+    gp = np.zeros(3) if grad_params is None else np.asarray(grad_params, dtype=np.float64)
+    if gp.shape != (3,):  # the library writes exactly three values
+        raise ValueError("grad_params must hold 3 values (hgcoef, qlc_monoq, qqc_monoq)")
+    buf = np.ascontiguousarray(gp)
     if shadow is not None:
         for f in SHADOW_FIELDS:
             if f in shadow:
                 set_shadow(domain, f, shadow[f])
-    gp = np.zeros(3) if grad_params is None else np.asarray(grad_params, dtype=np.float64)
-    buf = np.ascontiguousarray(gp)
     _check(domain._lib, domain._ctx,
            domain._lib.lulesh_vjp_cycle(domain._ctx, buf.ctypes.data_as(_lib.P_double)))
     if grad_params is not None:

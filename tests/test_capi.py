@@ -70,3 +70,12 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(ImportError):
         _lib.load()
+
+
+def test_vjp_grad_params_size_checked_before_the_library():
+    """The library writes three parameter gradients; a shorter buffer is
+    refused on the host before any call."""
+    from lulesh_amd import vjp_cycle
+    for bad in ([1.0, 2.0], [0.0] * 4, [[0.0] * 3]):
+        with pytest.raises(ValueError):
+            vjp_cycle(object(), None, bad)
