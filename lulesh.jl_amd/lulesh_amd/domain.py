@@ -380,9 +380,12 @@ class Batch:
 
     def __init__(self, nx, einit_scale=None, hgcoef=None, nr=None, quirks=QUIRKS_REFERENCE,
                  device=0):
+        n = len(nx)
+        for name, arr in (("einit_scale", einit_scale), ("hgcoef", hgcoef), ("nr", nr)):
+            if arr is not None and len(arr) != n:
+                raise ValueError(f"{name} must have one entry per instance ({n})")
         lib = _lib.load()
         self._lib = lib
-        n = len(nx)
         nx_a = (ctypes.c_int32 * n)(*[int(v) for v in nx])
         sc_a = (ctypes.c_double * n)(*[float(v) for v in einit_scale]) if einit_scale is not None else None
         hg_a = (ctypes.c_double * n)(*[float(v) for v in hgcoef]) if hgcoef is not None else None

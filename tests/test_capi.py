@@ -79,3 +79,15 @@ def test_vjp_grad_params_size_checked_before_the_library():
     for bad in ([1.0, 2.0], [0.0] * 4, [[0.0] * 3]):
         with pytest.raises(ValueError):
             vjp_cycle(object(), None, bad)
+
+
+def test_batch_parameter_lengths_checked_before_the_library():
+    """Per-instance parameter lists must match the instance count (a short
+    hgcoef list would otherwise reach the library zero-padded)."""
+    from lulesh_amd import Batch
+    with pytest.raises(ValueError):
+        Batch([8, 9], hgcoef=[3.0])
+    with pytest.raises(ValueError):
+        Batch([8, 9], einit_scale=[1.0, 1.0, 1.0])
+    with pytest.raises(ValueError):
+        Batch([8], nr=[1, 5])
