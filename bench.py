@@ -156,6 +156,23 @@ def bench_batched(n=64, seed=0, cap=5000):
             "stopped_by_error": int(sum(e != 0 for e in lone_err))}
 
 
+def bench_batch_cycles(k, n=64, seed=0):
+    """The batched workload's instances advanced k cycles together, all of
+    them still running (the first instance to stop does so at cycle 463)."""
+    from lulesh_amd import Batch, c5_instances
+    nx, scale, hg, nr = c5_instances(n, seed)
+    b = Batch(nx, scale, hg, nr)
+    b.run(2)
+    t0 = time.perf_counter()
+    cyc = b.run(2 + k)
+    wall = time.perf_counter() - t0
+    b.close()
+    if not all(int(c) == k for c in cyc):
+        raise RuntimeError("an instance stopped early")
+    return {"config": f"{n} Sedov instances (seed {seed}), {sum(v ** 3 for v in nx)} elements, "
+                      f"cycles 3-{2 + k}", "ms_per_cycle": round(wall * 1e3 / k, 4)}
+
+
 def cpu_baseline(nx, cycles, nthreads):
     import oracle as O
     od = O.OracleDomain(nx, O.QUIRKS_REFERENCE, nthreads)
@@ -184,6 +201,10 @@ def main():
     ap.add_argument("--only", choices=("adjoint", "batched"), default=None,
                     help="run only the adjoint-ratio or the batched-speedup measurement and print "
                          "its JSON object (for profiling them on their own)")
+    ap.add_argument("--batch-cycles", type=int, default=0, metavar="K",
+                    help="only advance the 64-instance batched workload K cycles with every "
+                         "instance live, after 2 warm-up cycles, and print its time per cycle "
+                         "(for profiling the batched kernels on their own)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the "
                          "reference point for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -210,6 +231,9 @@ def main():
         sys.exit(2)
     if args.only:
         print(json.dumps(bench_adjoint() if args.only == "adjoint" else bench_batched()))
+        return
+    if args.batch_cycles:
+        print(json.dumps(bench_batch_cycles(args.batch_cycles)))
         return
 
     from lulesh_amd import Domain, LuleshProblem, run
