@@ -233,9 +233,10 @@ int lulesh_set_timing(lulesh_ctx *ctx, int32_t on);
  * changes any result: every node sums its corners in the same order.
  * lulesh_autotune times k_nodal on the live state for five tile-width
  * ranges x the default and depths 2..32 (with its parity flip off the
- * kernel only writes the inactive node set, so the state is unchanged) and
- * keeps the fastest pair; *chosen (the depth of instance 0) and *best_ms
- * (per launch) may be NULL. */
+ * kernel only writes the inactive node set, so the state is unchanged; the
+ * fx.. and xdd.. fields of lulesh_set_keep_aux and every instance's error
+ * code also stay as they were) and keeps the fastest pair; *chosen (the
+ * depth of instance 0) and *best_ms (per launch) may be NULL. */
 int lulesh_set_nodal_depth(lulesh_ctx *ctx, int32_t depth);
 int32_t lulesh_nodal_depth(const lulesh_ctx *ctx);
 int lulesh_set_nodal_tiles(lulesh_ctx *ctx, int32_t lo, int32_t hi);

@@ -1840,11 +1840,17 @@ int lulesh_autotune(lulesh_ctx *c, int32_t *chosen, double *best_ms) {
     const int cand[] = {0, 2, 4, 8, 16, 32};
     int maxs = 0;
     for (const InstDesc &I : c->inst) maxs = I.s > maxs ? I.s : maxs;
+    // the tuning launches may raise an error code the live state would
+    // raise on its next cycle; the scalars are put back as they were, so
+    // an error already on the device survives and a new one does not appear
+    int rc = pull_scalars(c);
+    if (!rc) rc = finish(c, false);
+    if (rc) return rc;
+    const std::vector<DevScalars> saved(c->hsc, c->hsc + c->n);
     int best_f = -1, best_cz = -1;
     float best_t = 0.f;
     hipEvent_t ea = get_event(c), eb = get_event(c);
     if (!ea || !eb) return fail(c, LULESH_ERR_HIP, "hipEventCreate failed");
-    int rc = 0;
     for (int fi = 0; fi < (int)(sizeof(fam) / sizeof(fam[0])) && !rc; ++fi) {
       c->tile_lo = fam[fi][0];
       c->tile_hi = fam[fi][1];
@@ -1853,7 +1859,8 @@ int lulesh_autotune(lulesh_ctx *c, int32_t *chosen, double *best_ms) {
         rc = plan_nodal(c, cz);
         if (rc) break;
         Batch a = make_args(c, 0, 0);
-        a.flip = 0;
+        a.flip = 0;      // the current node set stays current
+        a.keep_aux = 0;  // fx.. and xdd.. keep the last real cycle's values
         hipError_t e = hipSuccess;
         for (int w = 0; w < 2 && e == hipSuccess; ++w) e = launch_kernel(KID_NODAL, c->stream, a);
         if (e == hipSuccess) e = hipEventRecord(ea, c->stream);
@@ -1880,8 +1887,11 @@ int lulesh_autotune(lulesh_ctx *c, int32_t *chosen, double *best_ms) {
     c->tile_hi = fam[best_f][1];
     c->depth_req = best_cz;
     rc = plan_nodal(c, best_cz);
-    if (!rc) rc = clear_err(c);
-    if (!rc) rc = finish(c, false);
+    if (rc) return rc;
+    std::copy(saved.begin(), saved.end(), c->hsc);
+    HIP_TRY(c, hipMemcpyAsync(c->b.sc, c->hsc, sizeof(DevScalars) * c->n, hipMemcpyHostToDevice,
+                              c->stream));
+    rc = finish(c, false);
     if (rc) return rc;
     if (chosen) *chosen = c->inst[0].cz;
     if (best_ms) *best_ms = best_t / 5.0;

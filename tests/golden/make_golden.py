@@ -4,9 +4,11 @@ of the final energy and x fields).  Run from the repo root after
 `make -C oracle`:  python tests/golden/make_golden.py
 
 With --large it writes tests/golden/oracle_runs_large.json instead: the
-full-size workloads (the s=192 headline's first 500 cycles, s=96 runs),
-SHA-256 of all eleven state fields; minutes of CPU time on 8 threads, so
-only the GPU tests read them back."""
+full-size workloads (the s=192 headline's first 500 cycles, s=96 runs) and
+the s=30 reference run into its stall, SHA-256 of all eleven state fields;
+minutes of CPU time on 8 threads, so only the GPU tests read them back.
+Case names after --large regenerate only those cases and keep the others
+already in the file:  python tests/golden/make_golden.py --large s30_reference_1700"""
 import hashlib
 import json
 import os
@@ -27,6 +29,11 @@ CASES = [  # (name, nx, quirks, num_iters)
 
 
 LARGE_CASES = [
+    # examples/benchmark.jl's loop at s=30 with the reference arithmetic: from
+    # cycle ~1350 dt decays geometrically and time stalls at 9.6672e-3, below
+    # stoptime, so the reference's `while time < stoptime` never ends there;
+    # the case pins the state 1700 cycles in, deep in the stall
+    ("s30_reference_1700", 30, 7, 1700),
     ("s192_first_500", 192, 7, 500),
     ("s96_first_2000", 96, 7, 2000),
     ("s96_quirk_free_to_stoptime", 96, 0, 10 ** 9),
@@ -65,13 +72,19 @@ def run_large(nx, quirks, num_iters, nthreads):
 
 def main():
     large = "--large" in sys.argv
+    only = [a for a in sys.argv[1:] if not a.startswith("--")]
+    fn = "oracle_runs_large.json" if large else "oracle_runs.json"
     out = {}
+    if only and os.path.exists(os.path.join(HERE, fn)):
+        with open(os.path.join(HERE, fn)) as f:
+            out = json.load(f)
     for name, nx, q, it in (LARGE_CASES if large else CASES):
+        if only and name not in only:
+            continue
         r = run_large(nx, q, it, os.cpu_count() or 8) if large else run_case(nx, q, it)
         r.update({"nx": nx, "quirks": q, "num_iters": it})
         out[name] = r
         print(name, r["cycle"], r["verify_printed"], flush=True)
-    fn = "oracle_runs_large.json" if large else "oracle_runs.json"
     with open(os.path.join(HERE, fn), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
 

@@ -98,8 +98,10 @@ def test_c5_batch_matches_lone_runs():
 def test_autotune_leaves_state_unchanged_and_results_identical():
     print("[autotune]", flush=True)
     b = Batch(NX, SCALE, HG)
+    b.instance(0).keep_aux(True)
     b.run(30)
-    before = {(i, f): b.instance(i).get(f) for i in range(len(NX)) for f in FIELDS}
+    aux = ("fx", "fy", "fz", "xdd", "ydd", "zdd", "ql", "qq", "dxx")
+    before = {(i, f): b.instance(i).get(f) for i in range(len(NX)) for f in FIELDS + aux}
     depth, ms = b.autotune()
     assert depth >= 1 and ms > 0  # instance 0's depth: the model's or a fixed one
     for i in range(len(NX)):  # a tile of one of the tuned width ranges
@@ -115,6 +117,19 @@ def test_autotune_leaves_state_unchanged_and_results_identical():
         same(b.instance(i), ref.instance(i), f"depth-independence {i}")
     b.close()
     ref.close()
+
+
+def test_autotune_keeps_error_codes():
+    """An instance stopped by a reference error before the tuning is still
+    reported so after it, and the others stay clean."""
+    print("[autotune-errors]", flush=True)
+    b = Batch(NX, SCALE, HG)
+    b.instance(2).set("qstop", -1.0)  # q = 0 > -1 at the first cycle
+    b.run(5, check=False)
+    assert b.errors() == [0, 0, 7, 0, 0]
+    b.autotune()
+    assert b.errors() == [0, 0, 7, 0, 0]
+    b.close()
 
 
 @pytest.mark.parametrize("lo,hi", [(16, 16), (12, 21), (8, 32), (24, 32), (2, 128)])
