@@ -283,94 +283,24 @@ int build_graph(lulesh_ctx *c) {
   return 0;
 }
 
-// k_nodal geometry: every instance marches `cz` node planes per workgroup.
-// cz <= 0 picks the default: the deepest of 16, 8, 4, 2 that still gives
-// the launch >= 1024 workgroups (small domains trade some recomputed halo
-// planes for parallelism).  Uploads the instance and block tables.
-// z-march depth of an instance: `depth` node planes per chunk, balanced so
-// that the chunks of a column are (nearly) equal
-int balanced_depth(int s, int depth) {
-  const int m = s + 1, chunks = (m + depth - 1) / depth;
-  return (m + chunks - 1) / chunks;
-}
-
-// Default z-march depth.  A k_nodal workgroup's time grows with the planes
-// it marches (depth + 1 element planes), and the launch runs in rounds of
-// (CUs x resident workgroups per CU): the estimate
-//   ceil(workgroups / slots) * (depth + 1)
-// is minimised over all depths (the smaller depth on ties, for balance).
-// At s=192 on 256 CUs x 2 this picks 3 chunks of 65 planes (507 workgroups,
-// one round) over 13 chunks of 15 (2197 workgroups, 4.3 rounds).
+// Default z-march depth for the context's instances (auto_depth, with a
+// round of CUs x resident k_nodal workgroups per CU).
 int auto_depth(const lulesh_ctx *c) {
-  int maxm = 2;
-  for (const InstDesc &I : c->inst) maxm = I.s + 1 > maxm ? I.s + 1 : maxm;
-  const int64_t slots = (int64_t)c->cus * c->nodal_per_cu;
-  int best = 2;
-  int64_t best_cost = -1;
-  for (int d = 1; d <= maxm; ++d) {
-    int64_t total = 0;
-    int deepest = 0;
-    for (const InstDesc &I : c->inst) {
-      const int cz = balanced_depth(I.s, d);
-      total += nodal_blocks(NodalTile{I.etx, I.ety, I.tx, I.ty}, I.s, cz);
-      deepest = cz > deepest ? cz : deepest;
-    }
-    const int64_t cost = ((total + slots - 1) / slots) * (deepest + 1);
-    if (best_cost < 0 || cost < best_cost) {
-      best = d;
-      best_cost = cost;
-    }
-  }
-  return best;
+  std::vector<NodalPlanInst> pi;
+  pi.reserve(c->inst.size());
+  for (const InstDesc &I : c->inst) pi.push_back({I.s, NodalTile{I.etx, I.ety, I.tx, I.ty}});
+  return lulesh::auto_depth(pi.data(), pi.size(), (int64_t)c->cus * c->nodal_per_cu);
 }
 
-// k_nodal dispatch order.  The dispatcher deals a launch's workgroups
-// round-robin over the eight XCDs (workgroup g to XCD g % 8), each with its
-// own L2.  XCD x gets the x-th of eight contiguous runs of the block order
-// (instance, chunk, tile row, tile), cut at equal sums of marched element
-// planes: neighbouring tiles, which read each other's halo nodes, share an
-// L2, and every XCD gets the same work.  Within a run the deepest chunks go
-// first (stable, so an instance's blocks stay together): the shallow ones
-// fill the launch's last round.  `live` (null: all) restricts the
-// table to the instances still running, so finished ones leave no hole in
-// one XCD's share.  The first call for a plan (glaunch 0) sizes the launch;
-// a later one that would need more workgroups than it has lists the live
-// blocks in plain order instead.  Synchronous.
+// k_nodal dispatch table (nodal_order) uploaded for the instances in
+// `live` (null: all), so finished instances leave no hole in one XCD's
+// share.  The first call for a plan (glaunch 0) sizes the launch.
+// Synchronous.
 int order_nodal(lulesh_ctx *c, const std::vector<char> *live) {
-  const std::vector<int> &gi = c->gblock_host;
-  std::vector<int> lv;
-  lv.reserve(gi.size());
-  int64_t total = 0;
-  for (int v = 0; v < (int)gi.size(); ++v)
-    if (!live || (*live)[gi[v]]) {
-      lv.push_back(v);
-      total += c->inst[gi[v]].cz + 1;
-    }
-  const int nl = (int)lv.size();
-  int cut[9];
-  cut[0] = 0;
-  int x = 1;
-  int64_t acc = 0;
-  for (int k = 0; k < nl; ++k) {
-    acc += c->inst[gi[lv[k]]].cz + 1;
-    while (x < 8 && acc * 8 >= total * x) cut[x++] = k + 1;
-  }
-  while (x <= 8) cut[x++] = nl;
-  int runlen = 0;
-  for (int r = 0; r < 8; ++r) runlen = cut[r + 1] - cut[r] > runlen ? cut[r + 1] - cut[r] : runlen;
-  if (c->b.glaunch == 0) c->b.glaunch = runlen * 8 > 8 ? runlen * 8 : 8;
-  std::vector<int> &order = c->gorder_host;
-  order.assign((size_t)c->b.glaunch, -1);
-  if (runlen * 8 <= c->b.glaunch) {
-    for (int r = 0; r < 8; ++r) {
-      std::stable_sort(lv.begin() + cut[r], lv.begin() + cut[r + 1], [&](int a, int b) {
-        return c->inst[gi[a]].cz > c->inst[gi[b]].cz;
-      });
-      for (int k = 0; k < cut[r + 1] - cut[r]; ++k) order[(size_t)k * 8 + r] = lv[cut[r] + k];
-    }
-  } else {
-    for (int k = 0; k < nl; ++k) order[k] = lv[k];
-  }
+  std::vector<int> cz(c->inst.size());
+  for (size_t i = 0; i < c->inst.size(); ++i) cz[i] = c->inst[i].cz;
+  c->gorder_host = nodal_order(c->gblock_host, cz, live, c->b.glaunch);
+  const std::vector<int> &order = c->gorder_host;
   c->nodal_ready = false;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if ((int64_t)order.size() > c->gorder_cap) {
@@ -985,53 +915,10 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
 }
 
 // ---------------------------------------------------------------------------
-// binomial checkpointing: Griewank, "Achieving logarithmic growth of temporal
-// and spatial complexity in reverse automatic differentiation" (Optimization
-// Methods and Software 1, 1992) and Griewank & Walther, "Algorithm 799:
-// revolve" (ACM TOMS 26, 2000).  With c saved states (the start one
-// included), reversing n steps needs at least
-//   t(n, c) = r n - beta(c + 1, r - 1)   forward steps besides the n
-// recording ones, r the least integer with beta(c, r) >= n,
-// beta(c, r) = C(c + r, c).
+// binomial checkpointing (revolve_cost, revolve_split and the recursion
+// revolve_reverse live in lulesh_plan.h): Revolve carries the schedule out
+// on the device
 // ---------------------------------------------------------------------------
-const int64_t kSat = (int64_t)1 << 62;
-
-int64_t beta_sat(int64_t c, int64_t r) {
-  if (r < 0) return 0;
-  unsigned __int128 v = 1;  // C(c + i, i) for i = 0..r, each step exact
-  for (int64_t i = 1; i <= r; ++i) {
-    v = v * (unsigned __int128)(c + i) / (unsigned __int128)i;
-    if (v >= (unsigned __int128)kSat) return kSat;
-  }
-  return (int64_t)v;
-}
-
-// fewest advancing forward steps that reverse n steps from a saved start
-// state with `spare` further states free
-int64_t revolve_cost(int64_t n, int64_t spare) {
-  if (n <= 1) return 0;
-  if (spare == 0) return n * (n - 1) / 2;
-  const int64_t c = spare + 1;
-  int64_t r = 0;
-  while (beta_sat(c, r) < n) ++r;
-  return r * n - beta_sat(c + 1, r - 1);
-}
-
-// where to save the next state: advance l steps (1 <= l < n).  The total
-// l + cost(n - l, spare - 1) + cost(l, spare) is convex in l (the cost is
-// piecewise linear in n with slopes r, non-decreasing), so the first l whose
-// forward difference is >= 0 minimises it.
-int64_t revolve_split(int64_t n, int64_t spare) {
-  auto g = [&](int64_t l) { return l + revolve_cost(n - l, spare - 1) + revolve_cost(l, spare); };
-  int64_t lo = 1, hi = n - 1;
-  while (lo < hi) {
-    const int64_t mid = lo + (hi - lo) / 2;
-    if (g(mid + 1) - g(mid) >= 0) hi = mid;
-    else lo = mid + 1;
-  }
-  return lo;
-}
-
 struct Revolve {
   lulesh_ctx *c;
   int64_t n;
@@ -1126,25 +1013,11 @@ struct Revolve {
     cur = -1;
     return rc;
   }
-  // reverse cycles b-1 .. a, the start state of cycle a saved, `spare`
-  // buffers free
-  int reverse(int64_t a0, int64_t b, int64_t spare) {
-    if (b - a0 == 1) return youturn(a0);
-    if (spare == 0) {
-      for (int64_t k = b - 1; k >= a0; --k) {
-        int rc = advance(a0, k);
-        if (!rc) rc = youturn(k);
-        if (rc) return rc;
-      }
-      return 0;
-    }
-    const int64_t m = a0 + revolve_split(b - a0, spare);
-    int rc = advance(a0, m);
-    if (!rc) rc = save(m);
-    if (!rc) rc = reverse(m, b, spare - 1);
-    if (rc) return rc;
-    pos[find(m)] = -1;
-    return reverse(a0, m, spare);
+  int drop(int64_t k) {
+    const int i = find(k);
+    if (i < 0) return fail(c, LULESH_ERR_HIP, "revolve: missing saved state");
+    pos[i] = -1;
+    return 0;
   }
 };
 
@@ -1178,7 +1051,7 @@ int grad_run_revolve(lulesh_ctx *c, int64_t ncycles, int64_t snapshots, double *
   }
   if (!rc) rc = R.save(0);
   R.cur = 0;
-  if (!rc) rc = R.reverse(0, ncycles, snapshots - 1);
+  if (!rc) rc = revolve_reverse(R, 0, ncycles, snapshots - 1);
   if (rc) return rc;
   if (dt_out) std::memcpy(dt_out, R.hdt.data(), sizeof(double) * ncycles);
   if (forward_cycles) *forward_cycles = R.forward;

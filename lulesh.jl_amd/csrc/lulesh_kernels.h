@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "lulesh_plan.h"
+
 namespace lulesh {
 
 constexpr int ERR_NONE = 0x7fffffff;
@@ -95,45 +97,6 @@ enum KernelId {
   KID_RUN_BEGIN = 6,
   KID_CLEAR_ERR = 7,
 };
-
-constexpr int BLOCK = 256;
-// k_nodal tiling: a workgroup marches a column of etx x ety element lanes
-// (etx * ety <= BLOCK) up `cz` node planes and owns the node columns whose
-// elements all lie in its column.  Along an axis of s elements (s+1 nodes),
-// with a lanes per tile: the first tile's elements start at node 0 and it
-// owns a nodes (no element precedes node 0); an inner tile's first element
-// is the halo shared with the tile before and it owns a-1 nodes; the last
-// tile owns the rest, at most a (no element follows node s); one tile covers
-// the axis when s+1 <= a.
-inline int nodal_span(int s, int a) {
-  const int m = s + 1;
-  if (m <= a) return 1;
-  const int rest = m - 2 * a;
-  return 2 + (rest > 0 ? (rest + a - 2) / (a - 1) : 0);
-}
-struct NodalTile {
-  int etx, ety, tx, ty;
-};
-// Per instance: the tile, etx in lo..hi lanes wide (by default 16..32: a
-// row's node loads span whole 128-byte lines), ety = BLOCK / etx, that
-// covers the (s+1)^2 node columns with the fewest tiles (the squarest among
-// equals).  At s=192 that is 17 x 15 (12 x 14 tiles); small domains avoid a
-// mostly idle last tile.  2 <= lo <= hi <= BLOCK / 2.
-inline NodalTile nodal_tile(int s, int lo = 16, int hi = 32) {
-  NodalTile best{0, 0, 0, 0};
-  for (int a = lo; a <= hi; ++a) {
-    const int b = BLOCK / a;
-    const int tx = nodal_span(s, a), ty = nodal_span(s, b);
-    const int dcand = a > b ? a - b : b - a;
-    const int dcur = best.etx > best.ety ? best.etx - best.ety : best.ety - best.etx;
-    if (best.etx == 0 || tx * ty < best.tx * best.ty || (tx * ty == best.tx * best.ty && dcand < dcur))
-      best = NodalTile{a, b, tx, ty};
-  }
-  return best;
-}
-inline int nodal_blocks(const NodalTile &t, int s, int cz) {
-  return t.tx * t.ty * ((s + 1 + cz - 1) / cz);
-}
 
 hipError_t launch_init(hipStream_t st, const Batch &b);
 // workgroups of k_nodal one CU holds at once (register and LDS limits)

@@ -1,0 +1,230 @@
+// Host-side planning arithmetic of liblulesh_hip, free of HIP so that a
+// CPU-only harness (built with the address and undefined-behaviour
+// sanitizers) compiles exactly this code:
+//   * the k_nodal tile of an instance (nodal_span, nodal_tile),
+//   * the z-march depth model (balanced_depth, auto_depth),
+//   * the XCD-aware k_nodal dispatch table (nodal_order),
+//   * the binomial checkpoint schedule of the adjoint (revolve_cost,
+//     revolve_split, revolve_reverse).
+#pragma once
+
+#include <stdint.h>
+
+#include <algorithm>
+#include <vector>
+
+namespace lulesh {
+
+constexpr int BLOCK = 256;
+
+// k_nodal tiling: a workgroup marches a column of etx x ety element lanes
+// (etx * ety <= BLOCK) up `cz` node planes and owns the node columns whose
+// elements all lie in its column.  Along an axis of s elements (s+1 nodes),
+// with a lanes per tile: the first tile's elements start at node 0 and it
+// owns a nodes (no element precedes node 0); an inner tile's first element
+// is the halo shared with the tile before and it owns a-1 nodes; the last
+// tile owns the rest, at most a (no element follows node s); one tile covers
+// the axis when s+1 <= a.
+inline int nodal_span(int s, int a) {
+  const int m = s + 1;
+  if (m <= a) return 1;
+  const int rest = m - 2 * a;
+  return 2 + (rest > 0 ? (rest + a - 2) / (a - 1) : 0);
+}
+struct NodalTile {
+  int etx, ety, tx, ty;
+};
+// Per instance: the tile, etx in lo..hi lanes wide (by default 16..32: a
+// row's node loads span whole 128-byte lines), ety = BLOCK / etx, that
+// covers the (s+1)^2 node columns with the fewest tiles (the squarest among
+// equals).  At s=192 that is 17 x 15 (12 x 14 tiles); small domains avoid a
+// mostly idle last tile.  2 <= lo <= hi <= BLOCK / 2.
+inline NodalTile nodal_tile(int s, int lo = 16, int hi = 32) {
+  NodalTile best{0, 0, 0, 0};
+  for (int a = lo; a <= hi; ++a) {
+    const int b = BLOCK / a;
+    const int tx = nodal_span(s, a), ty = nodal_span(s, b);
+    const int dcand = a > b ? a - b : b - a;
+    const int dcur = best.etx > best.ety ? best.etx - best.ety : best.ety - best.etx;
+    if (best.etx == 0 || tx * ty < best.tx * best.ty || (tx * ty == best.tx * best.ty && dcand < dcur))
+      best = NodalTile{a, b, tx, ty};
+  }
+  return best;
+}
+inline int nodal_blocks(const NodalTile &t, int s, int cz) {
+  return t.tx * t.ty * ((s + 1 + cz - 1) / cz);
+}
+
+// z-march depth of an instance: `depth` node planes per chunk, balanced so
+// that the chunks of a column are (nearly) equal
+inline int balanced_depth(int s, int depth) {
+  const int m = s + 1, chunks = (m + depth - 1) / depth;
+  return (m + chunks - 1) / chunks;
+}
+
+// one instance as the k_nodal planner sees it
+struct NodalPlanInst {
+  int s;
+  NodalTile t;
+};
+
+// Default z-march depth.  A k_nodal workgroup's time grows with the planes
+// it marches (depth + 1 element planes), and the launch runs in rounds of
+// `slots` (CUs x resident workgroups per CU): the estimate
+//   ceil(workgroups / slots) * (depth + 1)
+// is minimised over all depths (the smaller depth on ties, for balance).
+// At s=192 on 256 CUs x 2 this picks 3 chunks of 65 planes (507 workgroups,
+// one round) over 13 chunks of 15 (2197 workgroups, 4.3 rounds).
+inline int auto_depth(const NodalPlanInst *insts, size_t n, int64_t slots) {
+  int maxm = 2;
+  for (size_t i = 0; i < n; ++i) maxm = insts[i].s + 1 > maxm ? insts[i].s + 1 : maxm;
+  if (slots < 1) slots = 1;
+  int best = 2;
+  int64_t best_cost = -1;
+  for (int d = 1; d <= maxm; ++d) {
+    int64_t total = 0;
+    int deepest = 0;
+    for (size_t i = 0; i < n; ++i) {
+      const NodalPlanInst &I = insts[i];
+      const int cz = balanced_depth(I.s, d);
+      total += nodal_blocks(I.t, I.s, cz);
+      deepest = cz > deepest ? cz : deepest;
+    }
+    const int64_t cost = ((total + slots - 1) / slots) * (deepest + 1);
+    if (best_cost < 0 || cost < best_cost) {
+      best = d;
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+
+// k_nodal dispatch order.  The dispatcher deals a launch's workgroups
+// round-robin over the eight XCDs (workgroup g to XCD g % 8), each with its
+// own L2.  XCD x gets the x-th of eight contiguous runs of the block order
+// (instance, chunk, tile row, tile), cut at equal sums of marched element
+// planes: neighbouring tiles, which read each other's halo nodes, share an
+// L2, and every XCD gets the same work.  Within a run the deepest chunks go
+// first (stable, so an instance's blocks stay together): the shallow ones
+// fill the launch's last round.
+//   gblock_inst[v]  instance of k_nodal block v,
+//   inst_cz[i]      z-march depth of instance i,
+//   live            instances to list (null: all),
+//   glaunch         workgroups launched: 0 sizes it (a multiple of 8, at
+//                   least 8); a later call keeps it, and lists the live
+//                   blocks in plain order when they need more rounds.
+// Returns the table: block run by each launched workgroup, -1 for none.
+inline std::vector<int> nodal_order(const std::vector<int> &gblock_inst,
+                                    const std::vector<int> &inst_cz,
+                                    const std::vector<char> *live, int &glaunch) {
+  std::vector<int> lv;
+  lv.reserve(gblock_inst.size());
+  int64_t total = 0;
+  for (int v = 0; v < (int)gblock_inst.size(); ++v)
+    if (!live || (*live)[gblock_inst[v]]) {
+      lv.push_back(v);
+      total += inst_cz[gblock_inst[v]] + 1;
+    }
+  const int nl = (int)lv.size();
+  int cut[9];
+  cut[0] = 0;
+  int x = 1;
+  int64_t acc = 0;
+  for (int k = 0; k < nl; ++k) {
+    acc += inst_cz[gblock_inst[lv[k]]] + 1;
+    while (x < 8 && acc * 8 >= total * x) cut[x++] = k + 1;
+  }
+  while (x <= 8) cut[x++] = nl;
+  int runlen = 0;
+  for (int r = 0; r < 8; ++r) runlen = cut[r + 1] - cut[r] > runlen ? cut[r + 1] - cut[r] : runlen;
+  if (glaunch == 0) glaunch = runlen * 8 > 8 ? runlen * 8 : 8;
+  std::vector<int> order((size_t)glaunch, -1);
+  if (runlen * 8 <= glaunch) {
+    for (int r = 0; r < 8; ++r) {
+      std::stable_sort(lv.begin() + cut[r], lv.begin() + cut[r + 1], [&](int a, int b) {
+        return inst_cz[gblock_inst[a]] > inst_cz[gblock_inst[b]];
+      });
+      for (int k = 0; k < cut[r + 1] - cut[r]; ++k) order[(size_t)k * 8 + r] = lv[cut[r] + k];
+    }
+  } else {
+    for (int k = 0; k < nl; ++k) order[k] = lv[k];
+  }
+  return order;
+}
+
+// ---------------------------------------------------------------------------
+// binomial checkpointing: Griewank, "Achieving logarithmic growth of temporal
+// and spatial complexity in reverse automatic differentiation" (Optimization
+// Methods and Software 1, 1992) and Griewank & Walther, "Algorithm 799:
+// revolve" (ACM TOMS 26, 2000).  With c saved states (the start one
+// included), reversing n steps needs at least
+//   t(n, c) = r n - beta(c + 1, r - 1)   forward steps besides the n
+// recording ones, r the least integer with beta(c, r) >= n,
+// beta(c, r) = C(c + r, c).
+// ---------------------------------------------------------------------------
+constexpr int64_t kRevolveSat = (int64_t)1 << 62;
+
+inline int64_t beta_sat(int64_t c, int64_t r) {
+  if (r < 0) return 0;
+  unsigned __int128 v = 1;  // C(c + i, i) for i = 0..r, each step exact
+  for (int64_t i = 1; i <= r; ++i) {
+    v = v * (unsigned __int128)(c + i) / (unsigned __int128)i;
+    if (v >= (unsigned __int128)kRevolveSat) return kRevolveSat;
+  }
+  return (int64_t)v;
+}
+
+// fewest advancing forward steps that reverse n steps from a saved start
+// state with `spare` further states free
+inline int64_t revolve_cost(int64_t n, int64_t spare) {
+  if (n <= 1) return 0;
+  if (spare == 0) return n * (n - 1) / 2;
+  const int64_t c = spare + 1;
+  int64_t r = 0;
+  while (beta_sat(c, r) < n) ++r;
+  return r * n - beta_sat(c + 1, r - 1);
+}
+
+// where to save the next state: advance l steps (1 <= l < n).  The total
+// l + cost(n - l, spare - 1) + cost(l, spare) is convex in l (the cost is
+// piecewise linear in n with slopes r, non-decreasing), so the first l whose
+// forward difference is >= 0 minimises it.
+inline int64_t revolve_split(int64_t n, int64_t spare) {
+  auto g = [&](int64_t l) { return l + revolve_cost(n - l, spare - 1) + revolve_cost(l, spare); };
+  int64_t lo = 1, hi = n - 1;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (g(mid + 1) - g(mid) >= 0) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// Reverse steps b-1 .. a0 with the start state of step a0 saved and `spare`
+// state buffers free.  `act` carries out the schedule:
+//   advance(from, to)  run steps from..to-1 starting at the saved state of `from`,
+//   save(m)            keep the current state (that of step m) in a free buffer,
+//   drop(m)            free the buffer holding step m's state,
+//   youturn(k)         run step k once more keeping its temporaries, then its adjoint;
+// each returns 0 or an error code, which ends the recursion.
+template <typename Actions>
+int revolve_reverse(Actions &act, int64_t a0, int64_t b, int64_t spare) {
+  if (b - a0 == 1) return act.youturn(a0);
+  if (spare == 0) {
+    for (int64_t k = b - 1; k >= a0; --k) {
+      int rc = act.advance(a0, k);
+      if (!rc) rc = act.youturn(k);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  const int64_t m = a0 + revolve_split(b - a0, spare);
+  int rc = act.advance(a0, m);
+  if (!rc) rc = act.save(m);
+  if (!rc) rc = revolve_reverse(act, m, b, spare - 1);
+  if (!rc) rc = act.drop(m);
+  if (rc) return rc;
+  return revolve_reverse(act, a0, m, spare);
+}
+
+}  // namespace lulesh
