@@ -51,6 +51,9 @@ extern "C" {
 #define LULESH_ERR_HOST_MEMORY 103  /* a host allocation failed (no C++ exception
                                        ever leaves an entry point); the
                                        context's state is then unspecified */
+#define LULESH_ERR_SCRATCH 104      /* the domain does not fit the fixed scratch
+                                       array of lulesh_lagrange_leapfrog_noalloc
+                                       (nx > LULESH_NOALLOC_MAX_NX) */
 
 /* ---- arithmetic quirks of the reference --------------------------------
  * On by default so results match the reference; a caller may clear them to
@@ -120,6 +123,47 @@ int lulesh_run(lulesh_ctx *ctx, int64_t num_iters, int64_t *cycles_run);
 /* verifyAndWriteFinalOutput's numbers (src/utils.jl:124-152):
  * out = {e[1], MaxAbsDiff, TotalAbsDiff, MaxRelDiff} over plane 0 */
 int lulesh_verify(lulesh_ctx *ctx, double out[4]);
+
+/* ---- one cycle without allocation --------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc -- and that hold the Domain's fields themselves in
+ * device memory (a Julia GPUDomain of device arrays, for instance).
+ * lulesh_lagrange_leapfrog_noalloc runs one lagrangeLeapFrog
+ * (src/domain.jl:3067-3080), with the kernels lulesh_lagrange_leapfrog
+ * runs, on those fields in place.  It allocates nothing, on the device or
+ * the host: the cycle's scratch (the second node set, the kinematics
+ * temporaries, the dt partial minima, the block tables and the device-side
+ * scalars) lives in a fixed per-GPU array of LULESH_NOALLOC_SCRATCH_BYTES,
+ * reserved when the library's code is loaded on a GPU and held until the
+ * process ends.  nx above LULESH_NOALLOC_MAX_NX returns LULESH_ERR_SCRATCH.
+ * All calls on one GPU share that GPU's array and leave their contents in
+ * it (each call sets up what it reads first): calls that overlap on one GPU
+ * (from several host threads or streams) race; the caller serialises them.
+ *   dom      nx, quirks and device pointers to the Domain fields, 0-based,
+ *            numbered as above: element fields nx^3 doubles (elemBC: nx^3
+ *            uint32 masks), node fields (nx+1)^3 doubles; e p q v ss delv
+ *            vdov arealg and x y z xd yd zd are updated, the others read.
+ *   scalars  host array of the 29 Domain scalars in the order of the name
+ *            list of lulesh_get_scalar (cycle as a double): all are read,
+ *            and dtcourant (index 3) and dthydro (index 4) are written as
+ *            calcTimeConstraintsForElems leaves them.
+ *   stream   a hipStream_t (NULL: the null stream); the call returns after
+ *            synchronising it.
+ *   device   the HIP ordinal the pointers belong to (made current).
+ * Returns 0, the lowest physics error 1..9 raised (the fields are then
+ * undefined, as after the reference's exception), or LULESH_ERR_*. */
+#define LULESH_NOALLOC_SCRATCH_BYTES (32u << 20)
+#define LULESH_NOALLOC_MAX_NX 68
+#define LULESH_NUM_SCALARS 29
+typedef struct lulesh_device_domain {
+  int32_t nx, quirks;
+  double *e, *p, *q, *v, *volo, *delv, *vdov, *arealg, *ss, *elemMass;
+  uint32_t *elemBC;
+  double *x, *y, *z, *xd, *yd, *zd, *nodalMass;
+} lulesh_device_domain;
+int lulesh_lagrange_leapfrog_noalloc(const lulesh_device_domain *dom, double *scalars,
+                                     void *stream, int32_t device);
 
 /* ---- ragged batches -----------------------------------------------------
  * No reference counterpart (the reference runs one Domain per process).  A

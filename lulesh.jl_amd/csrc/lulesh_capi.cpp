@@ -1242,6 +1242,8 @@ const char *lulesh_error_string(int code) {
   if (code == LULESH_ERR_HIP) return "HIP runtime error";
   if (code == LULESH_ERR_UNSUPPORTED) return "unsupported";
   if (code == LULESH_ERR_HOST_MEMORY) return "host memory allocation failed";
+  if (code == LULESH_ERR_SCRATCH)
+    return "domain too large for the fixed scratch array (nx > LULESH_NOALLOC_MAX_NX)";
   return "unknown error";
 }
 

@@ -18,9 +18,24 @@ P_double = ctypes.POINTER(c_double)
 P_int64 = ctypes.POINTER(c_int64)
 P_uint32 = ctypes.POINTER(ctypes.c_uint32)
 P_int32 = ctypes.POINTER(ctypes.c_int32)
+LULESH_ERR_INVALID_ARG, LULESH_ERR_SCRATCH = 100, 104
+NOALLOC_MAX_NX = 68
+
+
+class DeviceDomain(ctypes.Structure):
+    """lulesh_device_domain: nx, quirks and device pointers to the Domain
+    fields (lulesh_lagrange_leapfrog_noalloc)."""
+    _fields_ = ([("nx", c_int32), ("quirks", c_int32)] +
+                [(n, P_double) for n in ("e", "p", "q", "v", "volo", "delv", "vdov", "arealg",
+                                         "ss", "elemMass")] +
+                [("elemBC", P_uint32)] +
+                [(n, P_double) for n in ("x", "y", "z", "xd", "yd", "zd", "nodalMass")])
+
 
 # name -> (restype, argtypes); mirrors include/lulesh_hip.h
 SIGNATURES = {
+    "lulesh_lagrange_leapfrog_noalloc": (c_int, [ctypes.POINTER(DeviceDomain), P_double, c_void_p,
+                                                 c_int32]),
     "lulesh_create": (c_int, [c_int32, c_int32, c_int32, c_int32, c_int32, c_int32,
                               ctypes.POINTER(c_void_p)]),
     "lulesh_destroy": (c_int, [c_void_p]),
