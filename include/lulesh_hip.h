@@ -67,10 +67,13 @@ typedef struct lulesh_ctx lulesh_ctx;
 
 /* Domain(prob::LuleshProblem) for one structured rank (src/domain.jl:467-771,
  * src/LULESH.jl:31-49): builds the Sedov problem with nx elements per edge
- * on the device.  nr/balance/cost are accepted as in LuleshProblem but, as in
- * the reference, never reach the arithmetic (the compute loops walk the
- * identity matElemlist, src/domain.jl:657-661).  `device` is a HIP ordinal.
- * nx must lie in 1..800 (LULESH_ERR_INVALID_ARG otherwise). */
+ * on the device.  nr/balance/cost are those of LuleshProblem: they define
+ * the region index sets (lulesh_region_download) and, as in the reference,
+ * never reach the arithmetic (the compute loops walk the identity
+ * matElemlist, src/domain.jl:657-661).  `device` is a HIP ordinal.  nx must
+ * lie in 1..800, nr in 1..LULESH_MAX_REGIONS, balance and cost >= 0 with
+ * sum_{r<=nr} r^balance within Int64 (LULESH_ERR_INVALID_ARG otherwise). */
+#define LULESH_MAX_REGIONS 4096
 int lulesh_create(int32_t nx, int32_t nr, int32_t balance, int32_t cost, int32_t quirks,
                   int32_t device, lulesh_ctx **out);
 int lulesh_destroy(lulesh_ctx *ctx);
@@ -123,6 +126,28 @@ int lulesh_run(lulesh_ctx *ctx, int64_t num_iters, int64_t *cycles_run);
 /* verifyAndWriteFinalOutput's numbers (src/utils.jl:124-152):
  * out = {e[1], MaxAbsDiff, TotalAbsDiff, MaxRelDiff} over plane 0 */
 int lulesh_verify(lulesh_ctx *ctx, double out[4]);
+
+/* ---- regions ------------------------------------------------------------
+ * createRegionIndexSets!(domain, nr, balance) of Domain(prob)
+ * (src/domain.jl:313-465), per instance, built on the host the first time
+ * one of the two calls below needs them.  lulesh_region_download copies one
+ * array as the reference stores it: "regNumList" (numElem region numbers,
+ * 1-based), "regElemlist" (numElem element ids, 1-based, grouped by region
+ * in sortRegions' order), "regCSR" (nr running end offsets, as the fill loop
+ * leaves them), "regReps" and "regSorted" (nr each, as sortRegions' partial
+ * bubble sort leaves them); n must equal the length.  The reference draws
+ * the regions with Julia 1.7's default generator seeded by Random.seed!(0)
+ * (xoshiro256++, seeded from the SHA-256 of the seed's words); this library
+ * restates that generator, so nr > 1 matches the reference only as far as
+ * that restatement does (nr = 1 draws nothing).
+ * lulesh_set_cost_mode(ctx, 1) switches on LULESH 2.0's load-imbalance cost
+ * model: k_qeos evaluates each element's energy update regReps times for
+ * its region (1, 1+cost or 10(1+cost)), as LULESH 2.0's EvalEOSForElems
+ * does; every result stays bit-identical, only the time changes.  Off by
+ * default; the first switch-on uploads 4 bytes per element. */
+int lulesh_region_download(lulesh_ctx *ctx, int32_t inst, const char *name, int64_t *dst,
+                           int64_t n);
+int lulesh_set_cost_mode(lulesh_ctx *ctx, int32_t on);
 
 /* ---- one cycle without allocation --------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device

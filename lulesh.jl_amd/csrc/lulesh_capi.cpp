@@ -21,6 +21,7 @@
 
 #include "lulesh_adjoint.h"
 #include "lulesh_kernels.h"
+#include "lulesh_regions.h"
 
 using namespace lulesh;
 
@@ -77,6 +78,11 @@ struct lulesh_ctx {
   std::vector<int> gorder_host;
   std::vector<char> order_live;  // instances the table holds (empty: all)
   AdjBuffers adj;
+  // region index sets per instance, built on first use (they never change
+  // and no default path reads them), and the cost mode's per-element reps
+  std::vector<RegionSets> regions;
+  int *elem_rep = nullptr;
+  int graph_cost_mode = -1;
 };
 
 namespace {
@@ -255,7 +261,8 @@ void release(lulesh_ctx *c) {
 int build_graph(lulesh_ctx *c) {
   if (!c->nodal_ready)
     return fail(c, LULESH_ERR_HIP, "k_nodal tables unavailable (a device allocation failed)");
-  if (c->graph && c->graph_keep_aux == c->keep_aux) return 0;
+  if (c->graph && c->graph_keep_aux == c->keep_aux && c->graph_cost_mode == c->b.cost_mode)
+    return 0;
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
@@ -280,6 +287,18 @@ int build_graph(lulesh_ctx *c) {
     return hip_fail(c, e, "hipGraphInstantiate");
   }
   c->graph_keep_aux = c->keep_aux;
+  c->graph_cost_mode = c->b.cost_mode;
+  return 0;
+}
+
+// createRegionIndexSets! for every instance (lulesh_regions.h), once
+int ensure_regions(lulesh_ctx *c) {
+  if (!c->regions.empty()) return 0;
+  std::vector<RegionSets> R((size_t)c->n);
+  for (int i = 0; i < c->n; ++i)
+    if (!build_regions(c->inst[i].E, c->nr[i], c->balance[i], c->cost[i], R[(size_t)i]))
+      return fail(c, LULESH_ERR_INVALID_ARG, "region index sets: invalid nr/balance/cost");
+  c->regions = std::move(R);
   return 0;
 }
 
@@ -391,6 +410,18 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
     // needs ~240 GB of HBM)
     if (nx[i] < 1 || nx[i] > 800) return LULESH_ERR_INVALID_ARG;
     if (einit_scale && !(einit_scale[i] > 0.0)) return LULESH_ERR_INVALID_ARG;
+    // regions: nr in 1..LULESH_MAX_REGIONS, balance and cost >= 0, and the
+    // cost denominator sum_r r^balance within Int64 (src/domain.jl:370-375)
+    if (nr && (nr[i] < 1 || nr[i] > LULESH_MAX_REGIONS)) return LULESH_ERR_INVALID_ARG;
+    if ((balance && balance[i] < 0) || (cost && cost[i] < 0)) return LULESH_ERR_INVALID_ARG;
+    {
+      const int64_t r = nr ? nr[i] : 1, b = balance ? balance[i] : 1;
+      int64_t w = 0, tot = 0;
+      for (int64_t k = 1; k <= r; ++k) {
+        if (!ipow_checked(k, b, &w) || tot > INT64_MAX - w) return LULESH_ERR_INVALID_ARG;
+        tot += w;
+      }
+    }
   }
   // all instances' nodes together below 2^31: block counts and the host's
   // block tables stay int-sized (far beyond what one GPU's memory holds)
@@ -1459,6 +1490,70 @@ int lulesh_get_scalar(lulesh_ctx *c, const char *name, double *value) {
 int lulesh_set_scalar(lulesh_ctx *c, const char *name, double value) {
   try {
     return lulesh_batch_set_scalar(c, 0, name, value);
+  } catch (...) {
+    return exception_status(c);
+  }
+}
+
+int lulesh_region_download(lulesh_ctx *c, int32_t inst, const char *name, int64_t *dst,
+                           int64_t n) {
+  try {
+    int rc = check_inst(c, inst);
+    if (rc) return rc;
+    if (!name || !dst) return fail(c, LULESH_ERR_INVALID_ARG, "null name or buffer");
+    rc = ensure_regions(c);
+    if (rc) return rc;
+    const RegionSets &R = c->regions[(size_t)inst];
+    const std::string s(name);
+    if (s == "regNumList" || s == "regElemlist") {
+      const std::vector<int32_t> &v = s == "regNumList" ? R.numList : R.elemList;
+      if (n != (int64_t)v.size()) return fail(c, LULESH_ERR_INVALID_ARG, "region field length mismatch");
+      for (int64_t i = 0; i < n; ++i) dst[i] = v[(size_t)i];
+      return 0;
+    }
+    const std::vector<int64_t> *v = s == "regCSR" ? &R.csr
+                                    : s == "regReps" ? &R.reps
+                                    : s == "regSorted" ? &R.sorted : nullptr;
+    if (!v) return fail(c, LULESH_ERR_INVALID_ARG, "unknown region field " + s);
+    if (n != (int64_t)v->size()) return fail(c, LULESH_ERR_INVALID_ARG, "region field length mismatch");
+    std::copy(v->begin(), v->end(), dst);
+    return 0;
+  } catch (...) {
+    return exception_status(c);
+  }
+}
+
+int lulesh_set_cost_mode(lulesh_ctx *c, int32_t on) {
+  try {
+    if (!c) return LULESH_ERR_INVALID_ARG;
+    if (!on) {
+      c->b.cost_mode = 0;
+      return 0;
+    }
+    if (!c->elem_rep) {
+      int rc = ensure_regions(c);
+      if (rc) return rc;
+      // an element's rep is its region's: regReps[regSorted[regNumList[e]]]
+      std::vector<int> rep((size_t)c->E_total);
+      for (int i = 0; i < c->n; ++i) {
+        const RegionSets &R = c->regions[(size_t)i];
+        const int64_t e0 = c->inst[i].e0;
+        for (int64_t e = 0; e < c->inst[i].E; ++e) {
+          const int64_t r = R.sorted[(size_t)R.numList[(size_t)e] - 1];
+          const int64_t k = R.reps[(size_t)r - 1];
+          rep[(size_t)(e0 + e)] = (int)(k < 1 ? 1 : (k > INT32_MAX ? INT32_MAX : k));
+        }
+      }
+      HIP_TRY(c, hipSetDevice(c->device));
+      int *d = nullptr;
+      rc = dev_alloc(c, &d, c->E_total);
+      if (rc) return rc;
+      HIP_TRY(c, hipMemcpy(d, rep.data(), sizeof(int) * rep.size(), hipMemcpyHostToDevice));
+      c->elem_rep = d;
+    }
+    c->b.elem_rep = c->elem_rep;
+    c->b.cost_mode = 1;
+    return 0;
   } catch (...) {
     return exception_status(c);
   }

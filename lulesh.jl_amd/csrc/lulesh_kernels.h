@@ -85,6 +85,8 @@ struct Batch {
   double *dt_record;        // k_time_increment stores each instance-0 dt here
   int64_t dt_base;          // ... at index cycle - 1 - dt_base (if non-null)
   int64_t num_iters;
+  int cost_mode;            // k_qeos repeats each element's energy update
+  const int *elem_rep;      // ... elem_rep[element] times (its region's regReps)
 };
 
 enum KernelId {

@@ -759,6 +759,7 @@ LDEV void block_min2(double &a, double &b) {
 // time; the reference's element-wise temporaries never mix elements),
 // updateVolumesForElems (2912-2930) and the per-element terms of
 // calcCourantConstraintForElems / calcHydroConstraintForElems (2964-3054)
+template <bool COST>
 __global__ __launch_bounds__(BLOCK) void k_qeos(Batch B) {
   LULESH_ELEM_PREAMBLE_G(B, QEOS_GROUP)
   if (B.gated && !sc->active) return;  // uniform over the block
@@ -853,11 +854,25 @@ __global__ __launch_bounds__(BLOCK) void k_qeos(Batch B) {
     if (eosvmin != 0.0 && vnewc < eosvmin) vnewc = eosvmin;
     if (eosvmax != 0.0 && vnewc > eosvmax) vnewc = eosvmax;
 
-    const double delvc = f.delv[e], e_old = f.e[e];
-    double p_old = f.p[e];
+    const double delvc_in = f.delv[e], e_old_in = f.e[e], p_old_in = f.p[e];
     const double TINY1 = 0.111111e-36, TINY3 = 0.333333e-18, SIXTH = 1.0 / 6.0;
     const double pmin = sc->pmin, p_cut = sc->p_cut, e_cut = sc->e_cut, q_cut = sc->q_cut;
     const double emin = sc->emin, rho0 = sc->refdens;
+    // LULESH-2.0 cost mode: the energy update of an element is evaluated as
+    // many times as its region's regReps says (the same inputs each time, so
+    // the same result); the empty asm statements make every evaluation real
+    int rep = 1;
+    if constexpr (COST) rep = B.elem_rep[I.e0 + e];
+    double p_new = 0.0, e_new = 0.0, q_new = 0.0, bvc = 0.0, pbvc = 0.0;
+    for (int rr = 0; rr < rep; ++rr) {
+    double delvc = delvc_in, e_old = e_old_in, p_old = p_old_in, q_o = q_old, vn = vnewc,
+           ql_r = qlin, qq_r = qquad;
+    if constexpr (COST) {
+      __asm__ volatile("" : "+v"(delvc), "+v"(e_old), "+v"(p_old), "+v"(q_o));
+      __asm__ volatile("" : "+v"(vn), "+v"(ql_r), "+v"(qq_r));
+    }
+    {
+    const double q_old = q_o, vnewc = vn, qlin = ql_r, qquad = qq_r;
     double compression = 1.0 / vnewc - 1.0;
     const double vchalf = vnewc - delvc * 0.5;
     double compHalfStep = (1.0 / vchalf) - 1.0;
@@ -868,12 +883,10 @@ __global__ __launch_bounds__(BLOCK) void k_qeos(Batch B) {
       compHalfStep = 0.0;
     }
     const double work = 0.0;
-    double bvc, pbvc;
-    double e_new = (e_old - ((0.5 * delvc) * (p_old + q_old))) + (0.5 * work);
+    e_new = (e_old - ((0.5 * delvc) * (p_old + q_old))) + (0.5 * work);
     if (e_new < emin) e_new = emin;
     const double pHalfStep = pressure(bvc, pbvc, e_new, compHalfStep, vnewc, pmin, p_cut, eosvmax);
     const double vhalf = 1.0 / (1.0 + compHalfStep);
-    double q_new;
     if (delvc > 0.0) {
       q_new = 0.0;
     } else {
@@ -885,7 +898,7 @@ __global__ __launch_bounds__(BLOCK) void k_qeos(Batch B) {
     e_new = e_new + 0.5 * work;
     if (fabs(e_new) < e_cut) e_new = 0.0;
     if (e_new < emin) e_new = emin;
-    double p_new = pressure(bvc, pbvc, e_new, compression, vnewc, pmin, p_cut, eosvmax);
+    p_new = pressure(bvc, pbvc, e_new, compression, vnewc, pmin, p_cut, eosvmax);
     double q_tilde;
     if (delvc > 0.0) {
       q_tilde = 0.0;
@@ -905,6 +918,12 @@ __global__ __launch_bounds__(BLOCK) void k_qeos(Batch B) {
       ssc = (ssc <= TINY1) ? TINY3 : sqrt(ssc);
       q_new = (ssc * qlin) + qquad;
       if (fabs(q_new) < q_cut) q_new = 0.0;
+    }
+    }
+    if constexpr (COST) {
+      __asm__ volatile("" ::"v"(p_new), "v"(e_new), "v"(q_new));
+      __asm__ volatile("" ::"v"(bvc), "v"(pbvc));
+    }
     }
     double ss = ((pbvc * e_new) + (((vnewc * vnewc) * bvc) * p_new)) / rho0;
     ss = (ss <= TINY1) ? TINY3 : sqrt(ss);
@@ -1081,7 +1100,12 @@ hipError_t launch_kernel(int id, hipStream_t st, const Batch &b) {
                          dim3(KIN_BLOCK), 0, st, b);
       break;
     case KID_QEOS:
-      hipLaunchKernelGGL(k_qeos, dim3(grouped_grid(b.eblocks, QEOS_GROUP)), dim3(BLOCK), 0, st, b);
+      if (b.cost_mode)
+        hipLaunchKernelGGL(k_qeos<true>, dim3(grouped_grid(b.eblocks, QEOS_GROUP)), dim3(BLOCK), 0,
+                           st, b);
+      else
+        hipLaunchKernelGGL(k_qeos<false>, dim3(grouped_grid(b.eblocks, QEOS_GROUP)), dim3(BLOCK),
+                           0, st, b);
       break;
     case KID_DT:
       hipLaunchKernelGGL(k_dt, dim3(b.n), dim3(1024), 0, st, b);

@@ -38,6 +38,8 @@ SIGNATURES = {
                                                  c_int32]),
     "lulesh_create": (c_int, [c_int32, c_int32, c_int32, c_int32, c_int32, c_int32,
                               ctypes.POINTER(c_void_p)]),
+    "lulesh_region_download": (c_int, [c_void_p, c_int32, c_char_p, P_int64, c_int64]),
+    "lulesh_set_cost_mode": (c_int, [c_void_p, c_int32]),
     "lulesh_destroy": (c_int, [c_void_p]),
     "lulesh_reset": (c_int, [c_void_p]),
     "lulesh_num_elem": (c_int64, [c_void_p]),

@@ -118,6 +118,7 @@ class Domain:
             object.__setattr__(d, k, int(nx))
         object.__setattr__(d, "numElem", int(batch._lib.lulesh_batch_num_elem(batch._ctx, inst)))
         object.__setattr__(d, "numNode", int(batch._lib.lulesh_batch_num_node(batch._ctx, inst)))
+        object.__setattr__(d, "numReg", int(batch.nr[inst]))
         object.__setattr__(d, "comm", None)
         return d
 
@@ -217,6 +218,23 @@ class Domain:
     def keep_aux(self, on=True):
         """Also write fx, xdd, dxx, ql, qq ... (off by default for speed)."""
         _check(self._lib, self._ctx, self._lib.lulesh_set_keep_aux(self._ctx, 1 if on else 0))
+
+    # -- regions (createRegionIndexSets!, src/domain.jl:338-465) --
+    def region(self, name):
+        """regNumList / regElemlist (numElem entries) or regCSR / regReps /
+        regSorted (numReg entries), as the reference stores them (1-based
+        region numbers and element ids)."""
+        n = self.numElem if name in ("regNumList", "regElemlist") else self.numReg
+        out = np.empty(n, dtype=np.int64)
+        _check(self._lib, self._ctx, self._lib.lulesh_region_download(
+            self._ctx, self._inst, name.encode(), out.ctypes.data_as(_lib.P_int64), n))
+        return out
+
+    def set_cost_mode(self, on=True):
+        """LULESH 2.0's load-imbalance model: each element's energy update
+        is evaluated regReps times for its region (whole context; results
+        unchanged)."""
+        _check(self._lib, self._ctx, self._lib.lulesh_set_cost_mode(self._ctx, 1 if on else 0))
 
     # -- measurement --
     def set_timing(self, on=True):
@@ -400,6 +418,7 @@ class Batch:
         self.quirks = int(quirks)
         self.n = n
         self.nx = [int(v) for v in nx]
+        self.nr = [int(v) for v in nr] if nr is not None else [1] * n
         self.views = [Domain._view(self, i, self.nx[i]) for i in range(n)]
 
     def instance(self, i):
@@ -463,6 +482,9 @@ class Batch:
 
     def keep_aux(self, on=True):
         _check(self._lib, self._ctx, self._lib.lulesh_set_keep_aux(self._ctx, 1 if on else 0))
+
+    def set_cost_mode(self, on=True):
+        _check(self._lib, self._ctx, self._lib.lulesh_set_cost_mode(self._ctx, 1 if on else 0))
 
 
 def c5_instances(n=64, seed=0):

@@ -416,6 +416,9 @@ struct oracle_domain {
   // node
   std::vector<double> x, y, z, xd, yd, zd, xdd, ydd, zdd, fx, fy, fz, nodalMass;
   std::vector<int64_t> symmX, symmY, symmZ, nodeElemCount, nodeElemStart, nodeElemCornerList;
+  // regions (src/types.jl:151-159): never read by the cycle
+  int64_t numReg = 0, balance = 0, cost = 0;
+  std::vector<int64_t> regNumList, regElemlist, regCSR, regReps, regSorted;
   // scratch (the reference allocates these per call)
   std::vector<double> sig, determ, fx_elem, fy_elem, fz_elem;
   std::vector<double> dvdx, dvdy, dvdz, x8n, y8n, z8n;
@@ -536,10 +539,162 @@ void setup_boundary_conditions(oracle_domain *d, int64_t edgeElems) {
   }
 }
 
+// Julia 1.7's default RNG as Random.seed!(0) leaves it: xoshiro256++
+// (Blackman & Vigna) whose four state words are the SHA-256 digest of the
+// seed's 32-bit words (make_seed(0) = UInt32[0], so four zero bytes), read
+// little-endian.  rand(Int) reinterprets the 64-bit output as signed.
+struct JuliaRNG {
+  uint64_t s[4] = {0xdb2fa90498613fdfull, 0x48d73dc42d195740ull, 0x8c49bc52dc8a77eaull,
+                   0x1911b814c02405e8ull};
+  int64_t rand_int() {
+    uint64_t s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
+    uint64_t tmp = s0 + s3;
+    uint64_t res = ((tmp << 23) | (tmp >> 41)) + s0;
+    uint64_t t = s1 << 17;
+    s2 ^= s0;
+    s3 ^= s1;
+    s1 ^= s2;
+    s0 ^= s3;
+    s2 ^= t;
+    s3 = (s3 << 45) | (s3 >> 19);
+    s[0] = s0; s[1] = s1; s[2] = s2; s[3] = s3;
+    return (int64_t)res;
+  }
+};
+
+// sortRegions, src/domain.jl:313-336 (1-based arrays as offsets - 1)
+void sort_regions(std::vector<int64_t> &regReps, std::vector<int64_t> &regSorted,
+                  std::vector<int64_t> &regElemSize, int64_t numReg) {
+  std::vector<int64_t> regIndex(numReg);
+  for (int64_t v = 1; v <= numReg; ++v) regIndex[v - 1] = v;
+  for (int64_t i = 1; i <= numReg - 1; ++i) {
+    for (int64_t j = 1; j <= numReg - i - 1; ++j) {
+      if (regReps[j - 1] < regReps[j]) {
+        int64_t temp = regReps[j - 1];
+        regReps[j - 1] = regReps[j];
+        regReps[j] = temp;
+        temp = regElemSize[j - 1];
+        regElemSize[j - 1] = regElemSize[j];
+        regElemSize[j] = temp;
+        temp = regIndex[j - 1];
+        regIndex[j - 1] = regIndex[j];
+        regIndex[j] = temp;
+      }
+    }
+  }
+  for (int64_t i = 1; i <= numReg; ++i) regSorted[regIndex[i - 1] - 1] = i;
+}
+
+// createRegionIndexSets!, src/domain.jl:338-465, for myRank = 0
+bool create_region_index_sets(oracle_domain *d, int64_t nr, int64_t b, int64_t cost) {
+  d->numReg = nr;
+  d->balance = b;
+  d->cost = cost;
+  const int64_t numReg = nr, numElem = d->numElem, myRank = 0;
+  JuliaRNG rng;
+  std::vector<int64_t> regElemSize(numReg);
+  int64_t nextIndex = 0;
+  d->regNumList.assign(numElem, 0);
+  d->regElemlist.assign(numElem, 0);
+  d->regCSR.assign(numReg, 0);
+  d->regReps.assign(numReg, 0);
+  d->regSorted.assign(numReg, 0);
+  if (numReg == 1) {
+    while (nextIndex < numElem) {
+      d->regNumList[nextIndex] = 1;
+      nextIndex += 1;
+    }
+    regElemSize[0] = 0;
+  } else {
+    int64_t lastReg = -1, runto = 0, costDenominator = 0;
+    std::vector<int64_t> regBinEnd(numReg);
+    for (int64_t i = 1; i <= numReg; ++i) {
+      regElemSize[i - 1] = 0;
+      int64_t w = 1;
+      for (int64_t k = 0; k < b; ++k) {
+        if (w > INT64_MAX / i) return false;  // beyond Int64
+        w *= i;
+      }
+      if (costDenominator > INT64_MAX - w) return false;
+      costDenominator += w;
+      regBinEnd[i - 1] = costDenominator;
+    }
+    while (nextIndex < numElem) {
+      int64_t regionVar = rng.rand_int() % costDenominator;
+      int64_t i = 0;
+      while (regionVar >= regBinEnd[i]) i += 1;
+      int64_t regionNum = ((i + myRank) % numReg) + 1;
+      while (regionNum == lastReg) {
+        regionVar = rng.rand_int() % costDenominator;
+        i = 0;
+        while (regionVar >= regBinEnd[i]) i += 1;
+        regionNum = ((i + myRank) % numReg) + 1;
+      }
+      int64_t binSize = rng.rand_int() % 1000;
+      int64_t elements;
+      if (binSize < 773) {
+        elements = rng.rand_int() % 15 + 1;
+      } else if (binSize < 937) {
+        elements = rng.rand_int() % 16 + 16;
+      } else if (binSize < 970) {
+        elements = rng.rand_int() % 32 + 32;
+      } else if (binSize < 974) {
+        elements = rng.rand_int() % 64 + 64;
+      } else if (binSize < 978) {
+        elements = rng.rand_int() % 128 + 128;
+      } else if (binSize < 981) {
+        elements = rng.rand_int() % 256 + 256;
+      } else {
+        elements = rng.rand_int() % 1537 + 512;
+      }
+      runto = elements + nextIndex;
+      while (nextIndex < runto && nextIndex < numElem) {
+        d->regNumList[nextIndex] = regionNum;
+        nextIndex += 1;
+      }
+      lastReg = regionNum;
+    }
+  }
+  for (int64_t i = 1; i <= numElem; ++i) {
+    int64_t r = d->regNumList[i - 1];
+    regElemSize[r - 1] += 1;
+  }
+  for (int64_t r = 1; r <= numReg; ++r) {
+    int64_t rep;
+    if (r < numReg / 2) {
+      rep = 1;
+    } else if (r < (numReg - (numReg + 15) / 20)) {
+      rep = 1 + cost;
+    } else {
+      rep = 10 * (1 + cost);
+    }
+    d->regReps[r - 1] = rep;
+  }
+  sort_regions(d->regReps, d->regSorted, regElemSize, numReg);
+  d->regCSR[0] = 0;
+  for (int64_t i = 2; i <= numReg; ++i) d->regCSR[i - 1] = d->regCSR[i - 2] + regElemSize[i - 2];
+  for (int64_t i = 1; i <= numElem; ++i) {
+    int64_t r = d->regSorted[d->regNumList[i - 1] - 1];
+    d->regElemlist[d->regCSR[r - 1]] = i;
+    d->regCSR[r - 1] += 1;
+  }
+  return true;
+}
+
 }  // namespace
 
+extern "C" void oracle_julia_rng_state(uint64_t out[4]) {
+  JuliaRNG r;
+  for (int k = 0; k < 4; ++k) out[k] = r.s[k];
+}
+
 extern "C" oracle_domain *oracle_create(int nx, int quirks, int nthreads) {
-  if (nx < 1) return nullptr;
+  return oracle_create_regions(nx, quirks, nthreads, 11, 1, 1);
+}
+
+extern "C" oracle_domain *oracle_create_regions(int nx, int quirks, int nthreads, int nr,
+                                                int balance, int cost) {
+  if (nx < 1 || nr < 1 || balance < 0 || cost < 0) return nullptr;
   oracle_domain *d = new oracle_domain();
   d->quirks = quirks;
   d->nthreads = nthreads < 1 ? 1 : nthreads;
@@ -646,6 +801,13 @@ extern "C" oracle_domain *oracle_create(int nx, int quirks, int nthreads) {
 
   setup_element_connectivities(d, edgeElems);
   setup_boundary_conditions(d, edgeElems);
+  // Domain(prob) builds the region index sets just before the symmetry
+  // planes, connectivities and boundary conditions (src/domain.jl:751-763);
+  // they share no data with those, and no step of the cycle reads them
+  if (!create_region_index_sets(d, nr, balance, cost)) {
+    delete d;
+    return nullptr;
+  }
 
   const int64_t E8 = 8 * E;
   d->sig.assign(E, 0.0);
@@ -1317,6 +1479,7 @@ extern "C" int64_t *oracle_ifield(oracle_domain *d, const char *name, int64_t *n
   if (false) {}
   F(matElemlist) F(nodelist) F(lxim) F(lxip) F(letam) F(letap) F(lzetam) F(lzetap)
   F(symmX) F(symmY) F(symmZ) F(nodeElemCount) F(nodeElemStart) F(nodeElemCornerList)
+  F(regNumList) F(regElemlist) F(regCSR) F(regReps) F(regSorted)
 #undef F
   if (!f) { if (n) *n = 0; return nullptr; }
   if (n) *n = (int64_t)f->size();

@@ -44,9 +44,19 @@ extern "C" {
 
 typedef struct oracle_domain oracle_domain;
 
-/* Domain(prob) for one structured rank, nr/balance/cost have no effect on
+/* Domain(prob) for one structured rank; nr/balance/cost have no effect on
  * the arithmetic (regions are never read by the compute path). */
 oracle_domain *oracle_create(int nx, int quirks, int nthreads);
+/* the same with the region index sets of createRegionIndexSets!
+ * (src/domain.jl:338-465) for nr regions, balance and cost (oracle_create
+ * uses the reference's defaults 11, 1, 1); exported by oracle_ifield as
+ * regNumList, regElemlist, regCSR, regReps, regSorted.  NULL for nr < 1 or
+ * negative balance/cost, or a cost denominator beyond Int64. */
+oracle_domain *oracle_create_regions(int nx, int quirks, int nthreads, int nr, int balance,
+                                     int cost);
+/* the state Random.seed!(0) gives Julia 1.7's xoshiro256++ generator, as
+ * the region builder uses it (the SHA-256 digest of four zero bytes) */
+void oracle_julia_rng_state(uint64_t out[4]);
 void oracle_destroy(oracle_domain *d);
 
 /* return 0 or one of the error codes below (same numbers as the product) */

@@ -28,6 +28,8 @@ def load():
     vp, i64, pi64 = ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)
     sig = {
         "oracle_create": (vp, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "oracle_create_regions": (vp, [ctypes.c_int] * 6),
+        "oracle_julia_rng_state": (None, [ctypes.POINTER(ctypes.c_uint64)]),
         "oracle_destroy": (None, [vp]),
         "oracle_time_increment": (ctypes.c_int, [vp]),
         "oracle_lagrange_leapfrog": (ctypes.c_int, [vp]),
@@ -74,10 +76,10 @@ def arr(a):
 class OracleDomain:
     """Domain(prob) of the CPU restatement (one structured rank)."""
 
-    def __init__(self, nx, quirks=QUIRKS_REFERENCE, nthreads=1):
+    def __init__(self, nx, quirks=QUIRKS_REFERENCE, nthreads=1, nr=11, balance=1, cost=1):
         self.L = load()
         self.nx = nx
-        self.h = self.L.oracle_create(nx, quirks, nthreads)
+        self.h = self.L.oracle_create_regions(nx, quirks, nthreads, nr, balance, cost)
         if not self.h:
             raise ValueError("oracle_create failed")
 
@@ -165,6 +167,12 @@ class OracleDomain:
 
 
 # element-level functions
+def julia_rng_state():
+    out = (ctypes.c_uint64 * 4)()
+    load().oracle_julia_rng_state(out)
+    return list(out)
+
+
 def cbrt(x):
     return load().oracle_cbrt(float(x))
 

@@ -1,0 +1,236 @@
+// Host checks of liblulesh_hip's planning code (lulesh.jl_amd/csrc/
+// lulesh_plan.h) and region builder (lulesh_regions.h), compiled with the
+// address and undefined-behaviour sanitizers by tests/host/Makefile and run
+// by tests/test_sanitize.py.  Every check is a property the kernels or the
+// adjoint driver rely on, evaluated over many more cases than the GPU
+// tests can run:
+//   * k_nodal tiles: along each axis the tiles own every node column once,
+//     and an owned column's elements are inside the tile's lanes;
+//   * auto_depth: its choice minimises the launch-round cost model (brute
+//     force over every depth);
+//   * nodal_order: every live block is dispatched exactly once, no finished
+//     instance's block is, and each XCD's run holds at most 1/8 of the work
+//     plus one block;
+//   * revolve_reverse: a recorded schedule reverses steps n-1..0 once each
+//     in that order, never needs more saved states than allowed, only
+//     advances from a saved state, and runs n + revolve_cost(n, c-1) steps;
+//   * build_regions: the product's arrays equal the oracle's.
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <set>
+#include <vector>
+
+#include "lulesh_oracle.h"
+#include "lulesh_plan.h"
+#include "lulesh_regions.h"
+
+using namespace lulesh;
+
+static int failures = 0;
+#define CHECK(cond)                                                  \
+  do {                                                               \
+    if (!(cond)) {                                                   \
+      if (failures < 20) std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, #cond); \
+      ++failures;                                                    \
+    }                                                                \
+  } while (0)
+
+// node columns owned by the tiles along one axis of s elements, a lanes a tile
+static void check_axis(int s, int a) {
+  const int n = nodal_span(s, a);
+  std::vector<int> owner(s + 1, 0);
+  for (int k = 0; k < n; ++k) {
+    const int i0 = k == 0 ? 0 : a + (k - 1) * (a - 1);
+    const int oi = k != 0;
+    const int iend = k == n - 1 ? s + 1 : i0 + a - oi;
+    CHECK(iend - i0 <= a);
+    for (int i = i0; i < iend; ++i) {
+      CHECK(i >= 0 && i <= s);
+      if (i >= 0 && i <= s) owner[i] += 1;
+      // the elements around node column i (i-1 and i) sit on lanes
+      // i-1-(i0-oi) and i-(i0-oi), both in [0, a)
+      const int first_lane = i0 - oi;
+      if (i >= 1) CHECK(i - 1 - first_lane >= 0 && i - 1 - first_lane < a);
+      if (i < s) CHECK(i - first_lane >= 0 && i - first_lane < a);
+    }
+  }
+  for (int i = 0; i <= s; ++i) CHECK(owner[i] == 1);
+}
+
+static void check_tiles() {
+  const int fam[][2] = {{16, 32}, {16, 16}, {12, 21}, {8, 32}, {24, 32}, {2, 128}};
+  for (const auto &f : fam)
+    for (int s = 1; s <= 800; ++s) {
+      const NodalTile t = nodal_tile(s, f[0], f[1]);
+      CHECK(t.etx >= f[0] && t.etx <= f[1] && t.etx * t.ety <= BLOCK);
+      CHECK(t.tx == nodal_span(s, t.etx) && t.ty == nodal_span(s, t.ety));
+      if (s <= 200 || s % 37 == 0) {
+        check_axis(s, t.etx);
+        check_axis(s, t.ety);
+      }
+    }
+}
+
+static void check_depth(std::mt19937 &rng) {
+  for (int trial = 0; trial < 300; ++trial) {
+    const int n = 1 + (int)(rng() % 8);
+    std::vector<NodalPlanInst> in;
+    int maxm = 2;
+    for (int i = 0; i < n; ++i) {
+      const int s = 1 + (int)(rng() % (trial % 3 == 0 ? 200 : 64));
+      in.push_back({s, nodal_tile(s)});
+      maxm = s + 1 > maxm ? s + 1 : maxm;
+    }
+    const int64_t slots = 1 + (int64_t)(rng() % 1024);
+    const int d = auto_depth(in.data(), in.size(), slots);
+    auto cost = [&](int dd) {
+      int64_t total = 0;
+      int deepest = 0;
+      for (const auto &I : in) {
+        const int cz = balanced_depth(I.s, dd);
+        CHECK(cz >= 1 && cz <= I.s + 1);
+        total += nodal_blocks(I.t, I.s, cz);
+        deepest = cz > deepest ? cz : deepest;
+      }
+      return ((total + slots - 1) / slots) * (deepest + 1);
+    };
+    CHECK(d >= 1 && d <= maxm);
+    const int64_t cd = cost(d);
+    for (int dd = 1; dd <= maxm; ++dd) CHECK(cd <= cost(dd));
+  }
+}
+
+static void check_order(std::mt19937 &rng) {
+  for (int trial = 0; trial < 300; ++trial) {
+    const int n = 1 + (int)(rng() % 40);
+    std::vector<int> gi, cz(n);
+    for (int i = 0; i < n; ++i) {
+      const int s = 1 + (int)(rng() % 64);
+      const NodalTile t = nodal_tile(s);
+      cz[i] = balanced_depth(s, 1 + (int)(rng() % 20));
+      gi.insert(gi.end(), nodal_blocks(t, s, cz[i]), i);
+    }
+    int glaunch = 0;
+    for (int pass = 0; pass < 3; ++pass) {
+      std::vector<char> live(n, 1);
+      if (pass > 0)
+        for (int i = 0; i < n; ++i) live[i] = (char)(rng() % 3 != 0);
+      const std::vector<int> ord = nodal_order(gi, cz, pass ? &live : nullptr, glaunch);
+      CHECK(glaunch >= 8 && glaunch % 8 == 0 && (int)ord.size() == glaunch);
+      std::vector<int> seen(gi.size(), 0);
+      int64_t total = 0, maxw = 0;
+      for (size_t v = 0; v < gi.size(); ++v)
+        if (live[gi[v]]) {
+          total += cz[gi[v]] + 1;
+          maxw = cz[gi[v]] + 1 > maxw ? cz[gi[v]] + 1 : maxw;
+        }
+      std::vector<int64_t> xcd(8, 0);
+      for (int g = 0; g < glaunch; ++g) {
+        const int v = ord[g];
+        if (v < 0) continue;
+        CHECK(v < (int)gi.size());
+        if (v >= (int)gi.size()) continue;
+        seen[v] += 1;
+        CHECK(live[gi[v]]);
+        xcd[g % 8] += cz[gi[v]] + 1;
+      }
+      for (size_t v = 0; v < gi.size(); ++v) CHECK(seen[v] == (live[gi[v]] ? 1 : 0));
+      // the first plan cuts the block order into eight equal-work runs
+      if (pass == 0)
+        for (int x = 0; x < 8; ++x) CHECK(xcd[x] * 8 <= total + 8 * maxw);
+    }
+  }
+}
+
+// records a revolve schedule
+struct Recorder {
+  int64_t n, snapshots;
+  std::set<int64_t> saved;  // states held (the start one included)
+  int64_t next_rev;         // the step the next youturn must be
+  int64_t forward = 0, peak = 0;
+  bool ok = true;
+  int advance(int64_t from, int64_t to) {
+    if (!saved.count(from) || to < from) ok = false;
+    forward += to - from;
+    return 0;
+  }
+  int save(int64_t m) {
+    if (saved.count(m)) ok = false;
+    saved.insert(m);
+    peak = (int64_t)saved.size() > peak ? (int64_t)saved.size() : peak;
+    return 0;
+  }
+  int drop(int64_t m) {
+    if (!saved.erase(m)) ok = false;
+    return 0;
+  }
+  int youturn(int64_t k) {
+    if (k != next_rev) ok = false;
+    next_rev = k - 1;
+    forward += 1;  // the step run again with its temporaries kept
+    return 0;
+  }
+};
+
+static void check_revolve() {
+  for (int64_t n = 1; n <= 150; ++n)
+    for (int64_t snaps = 1; snaps <= 12; ++snaps) {
+      const int64_t c = snaps < n ? snaps : n;
+      Recorder R{n, c, {0}, n - 1};
+      R.peak = 1;
+      revolve_reverse(R, 0, n, c - 1);
+      CHECK(R.ok);
+      CHECK(R.next_rev == -1);
+      CHECK(R.peak <= c);
+      CHECK(R.saved.size() == 1 && *R.saved.begin() == 0);
+      // the first descent of the schedule is the recording forward pass, so
+      // advancing steps + youturns = revolve_cost + n
+      CHECK(R.forward == n + revolve_cost(n, c - 1));
+    }
+  CHECK(revolve_cost(10000, 19) < 5 * 10000);
+  CHECK(beta_sat(1000, 1000) == kRevolveSat);
+}
+
+static void check_regions() {
+  const int cases[][4] = {{1, 1, 1, 1}, {2, 11, 1, 1}, {8, 11, 1, 1}, {8, 2, 1, 3},
+                          {12, 5, 0, 2}, {16, 16, 2, 1}, {20, 32, 1, 0}, {9, 7, 3, 4}};
+  for (const auto &k : cases) {
+    RegionSets R;
+    CHECK(build_regions((int64_t)k[0] * k[0] * k[0], k[1], k[2], k[3], R));
+    oracle_domain *d = oracle_create_regions(k[0], 7, 1, k[1], k[2], k[3]);
+    CHECK(d != nullptr);
+    if (!d) continue;
+    int64_t n = 0;
+    const int64_t *num = oracle_ifield(d, "regNumList", &n);
+    CHECK(n == (int64_t)R.numList.size());
+    for (int64_t i = 0; i < n; ++i) CHECK(num[i] == R.numList[i]);
+    const int64_t *el = oracle_ifield(d, "regElemlist", &n);
+    for (int64_t i = 0; i < n; ++i) CHECK(el[i] == R.elemList[i]);
+    const std::vector<int64_t> *small[] = {&R.csr, &R.reps, &R.sorted};
+    const char *names[] = {"regCSR", "regReps", "regSorted"};
+    for (int j = 0; j < 3; ++j) {
+      const int64_t *p = oracle_ifield(d, names[j], &n);
+      CHECK(n == (int64_t)small[j]->size());
+      for (int64_t i = 0; i < n; ++i) CHECK(p[i] == (*small[j])[i]);
+    }
+    oracle_destroy(d);
+  }
+  RegionSets R;
+  CHECK(!build_regions(8, 0, 1, 1, R));
+  CHECK(!build_regions(8, 16, 40, 1, R));
+  int64_t w = 0;
+  CHECK(ipow_checked(1, 1 << 30, &w) && w == 1);
+  CHECK(!ipow_checked(2, 63, &w) && ipow_checked(2, 62, &w));
+}
+
+int main() {
+  std::mt19937 rng(2024);
+  check_tiles();
+  check_depth(rng);
+  check_order(rng);
+  check_revolve();
+  check_regions();
+  std::printf("failures %d\n", failures);
+  return failures == 0 ? 0 : 1;
+}

@@ -22,7 +22,7 @@ def adj(tmp_path_factory):
     out = tmp_path_factory.mktemp("adj") / "libadj_host.so"
     subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
                     "-I", os.path.join(REPO, "lulesh.jl_amd", "csrc"),
-                    os.path.join(HERE, "adjoint_host", "adj_host.cpp"), "-o", str(out)],
+                    os.path.join(HERE, "host", "adj_host.cpp"), "-o", str(out)],
                    check=True)
     L = ctypes.CDLL(str(out))
     D = ctypes.c_double
