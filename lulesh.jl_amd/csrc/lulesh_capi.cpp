@@ -77,6 +77,12 @@ struct lulesh_ctx {
   int64_t gorder_cap = 0;
   std::vector<int> gorder_host;
   std::vector<char> order_live;  // instances the table holds (empty: all)
+  // k_kin / k_qeos work tables (kin_table, qeos_table), sized for every
+  // instance at create; a live subset rewrites them in place
+  std::vector<int> kin_base;       // band rotation per instance
+  std::vector<int> eblock_host;    // instance of each element block
+  KinWork *kin_work = nullptr;
+  int *qeos_order = nullptr;
   AdjBuffers adj;
   // region index sets per instance, built on first use (they never change
   // and no default path reads them), and the cost mode's per-element reps
@@ -342,6 +348,24 @@ int order_nodal(lulesh_ctx *c, const std::vector<char> *live) {
   return 0;
 }
 
+// the k_kin and k_qeos work tables for the instances in `live` (null: all),
+// uploaded into the tables create sized for all of them
+int order_elem(lulesh_ctx *c, const std::vector<char> *live) {
+  std::vector<int> s(c->inst.size());
+  for (size_t i = 0; i < s.size(); ++i) s[i] = c->inst[i].s;
+  int launch = c->b.kin_launch;
+  const std::vector<KinWork> kin = kin_table(s, live, c->kin_base, launch);
+  if (launch != c->b.kin_launch) return fail(c, LULESH_ERR_HIP, "k_kin work table size changed");
+  const std::vector<int> qeos = qeos_table(c->eblock_host, live);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpyAsync(c->kin_work, kin.data(), kin.size() * sizeof(KinWork),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(c->qeos_order, qeos.data(), qeos.size() * sizeof(int),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int plan_nodal(lulesh_ctx *c, int cz) {
   // until the new tables are on the device, k_nodal must not run, and the
   // captured graph (which holds the old ones) is gone
@@ -534,6 +558,21 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
   B.inst = dinst;
   B.eblock_inst = deb;
   B.nblock_inst = dnb;
+  // k_kin / k_qeos work tables, sized for all instances
+  c->eblock_host = std::move(eblock_inst);
+  if (!rc) {
+    std::vector<int> sz((size_t)n);
+    for (int i = 0; i < n; ++i) sz[(size_t)i] = c->inst[(size_t)i].s;
+    int launch = 0;
+    (void)kin_table(sz, nullptr, c->kin_base, launch);
+    B.kin_launch = launch;
+    B.qeos_launch = qeos_launch(eb);
+    rc = dev_alloc(c, &c->kin_work, launch);
+    if (!rc) rc = dev_alloc(c, &c->qeos_order, B.qeos_launch);
+    B.kin_work = c->kin_work;
+    B.qeos_order = c->qeos_order;
+    if (!rc) rc = order_elem(c, nullptr);
+  }
   if (!rc) rc = plan_nodal(c, 0);
   if (!rc) rc = lulesh_reset(c);
   if (rc) return rc;
@@ -612,8 +651,10 @@ int run_cycles_body(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
       live[i] = c->hsc[i].active != 0;
       all = all && live[i];
     }
+    // (and out of k_kin's and k_qeos' work tables)
     if (all ? !c->order_live.empty() : live != c->order_live) {
       rc = order_nodal(c, all ? nullptr : &live);
+      if (!rc) rc = order_elem(c, all ? nullptr : &live);
       if (rc) return rc;
     }
   }
@@ -635,7 +676,8 @@ int run_cycles_body(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run) {
 int run_cycles(lulesh_ctx *c, int64_t num_iters, int64_t *cycles_run /* [n] or null */) {
   int rc = run_cycles_body(c, num_iters, cycles_run);
   if (!c->order_live.empty()) {
-    const int r2 = order_nodal(c, nullptr);
+    int r2 = order_nodal(c, nullptr);
+    if (!r2) r2 = order_elem(c, nullptr);
     if (!rc) rc = r2;
   }
   return rc;

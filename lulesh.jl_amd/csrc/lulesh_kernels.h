@@ -72,8 +72,12 @@ struct Batch {
   const int *nblock_inst;   // instance of each node-kernel block
   const int *gblock_inst;   // instance of each k_nodal block
   const int *gblock_order;  // k_nodal block run by each launched workgroup, -1: none
+  const KinWork *kin_work;  // k_kin chunk run by each launched workgroup (kin_table)
+  const int *qeos_order;    // k_qeos block run by each launched workgroup, -1: none
   int eblocks, nblocks, gblocks;
   int glaunch;              // k_nodal workgroups launched (a multiple of 8)
+  int kin_launch;           // k_kin workgroups launched
+  int qeos_launch;          // k_qeos workgroups launched
   DevScalars *sc;           // [n]
   DevFields f;
   double *dtpart;           // 2 per element-kernel block

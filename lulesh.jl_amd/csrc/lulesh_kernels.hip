@@ -116,25 +116,14 @@ LDEV NodeSet pick(const DevFields &f, int k) {
 
 // Per-block instance lookup for the element-, node- and k_nodal-shaped
 // grids: every block belongs to exactly one instance (instances start on a
-// block boundary), so the lookup is uniform over the block.
-// XCD-grouped block order for the element kernels: the dispatcher deals
-// workgroups round-robin over the 8 XCDs, so hand each XCD runs of G
-// consecutive blocks (rows of neighbouring elements that share node rows in
-// its L2) while the eight XCDs still sweep nearby addresses together.  The
-// launch is padded to a multiple of 8G; -1 marks a padding workgroup.
-template <unsigned G>
-__device__ __forceinline__ int xcd_group_block(unsigned b, int nb) {
-  const unsigned x = b % 8u, k = b / 8u;
-  const unsigned v = ((k / G) * 8u + x) * G + k % G;
-  return v < (unsigned)nb ? (int)v : -1;
-}
-constexpr unsigned KIN_GROUP = 16, QEOS_GROUP = 4;
+// block boundary), so the lookup is uniform over the block.  k_kin and
+// k_qeos take their blocks from host-built tables (kin_table, qeos_table:
+// XCD-aware orders that leave finished instances out).
 // k_kin (no barriers, no LDS, three waves per SIMD by its registers) runs
-// each element block as KIN_SUB one-wave workgroups: a finished wave's slot
-// is refilled without waiting for a whole 4-wave workgroup to fit (at
-// s=192: 0.292 against 0.304 ms per launch)
-constexpr unsigned KIN_SUB = 4, KIN_BLOCK = BLOCK / KIN_SUB;
-inline unsigned grouped_grid(int nb, unsigned g) { return ((unsigned)nb + 8 * g - 1) / (8 * g) * (8 * g); }
+// one-wave workgroups of KIN_CHUNK elements: a finished wave's slot is
+// refilled without waiting for a whole 4-wave workgroup to fit (at s=192:
+// 0.292 against 0.304 ms per launch)
+static_assert(KIN_CHUNK == 64, "k_kin workgroups are one wave");
 
 #define LULESH_ELEM_PREAMBLE(B)                                                  \
   const int inst_ = (B).eblock_inst[blockIdx.x];                               \
@@ -144,8 +133,8 @@ inline unsigned grouped_grid(int nb, unsigned g) { return ((unsigned)nb + 8 * g 
   const int64_t E = I.E;                                                       \
   const int64_t e = (int64_t)(blockIdx.x - I.eb0) * BLOCK + threadIdx.x;       \
   const DevFields f = slice((B).f, I);
-#define LULESH_ELEM_PREAMBLE_G(B, G)                                             \
-  const int vb_ = xcd_group_block<G>(blockIdx.x, (B).eblocks);                 \
+#define LULESH_ELEM_PREAMBLE_T(B, TABLE)                                         \
+  const int vb_ = (B).TABLE[blockIdx.x];                                       \
   if (vb_ < 0) return;                                                         \
   const int inst_ = (B).eblock_inst[vb_];                                      \
   const InstDesc I = (B).inst[inst_];                                          \
@@ -617,20 +606,21 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
 
 // calcKinematicsForElems (src/domain.jl:2099-2147), the deviatoric loop
 // (2156-2171) and calcMonotonicQGradientsForElems (2175-2320)
-__global__ __launch_bounds__(KIN_BLOCK) void k_kin(Batch B) {
-  // XCD-grouped order over the sub-blocks (runs of KIN_GROUP element blocks)
-  const int vs_ = xcd_group_block<KIN_GROUP * KIN_SUB>(blockIdx.x, (B).eblocks * (int)KIN_SUB);
-  if (vs_ < 0) return;
-  const int vb_ = vs_ / (int)KIN_SUB;
-  const int inst_ = B.eblock_inst[vb_];
-  const InstDesc I = B.inst[inst_];
-  DevScalars *sc = B.sc + inst_;
-  const int s = I.s;
-  const int64_t E = I.E;
-  const int64_t e = (int64_t)(vb_ - I.eb0) * BLOCK + (vs_ % (int)KIN_SUB) * (int)KIN_BLOCK + threadIdx.x;
-  const DevFields f = slice(B.f, I);
-  if (e >= E) return;
+__global__ __launch_bounds__(KIN_CHUNK) void k_kin(Batch B) {
+  // the chunk of one element plane this workgroup runs: kin_table bands
+  // the planes so that an XCD's consecutive chunks climb one band of the
+  // domain plane by plane and share node rows in its L2
+  const KinWork w = B.kin_work[blockIdx.x];
+  if (w.inst < 0) return;
+  const InstDesc I = B.inst[w.inst];
+  DevScalars *sc = B.sc + w.inst;
   if (B.gated && !sc->active) return;
+  const int s = I.s;
+  const uint32_t plane = (uint32_t)s * (uint32_t)s;
+  const uint32_t plane_end = ((uint32_t)w.start / plane + 1u) * plane;
+  const int64_t e = (int64_t)w.start + threadIdx.x;
+  const DevFields f = slice(B.f, I);
+  if (e >= (int64_t)plane_end) return;
   const int keep_aux = B.keep_aux;
   uint32_t nid[4];
   {
@@ -761,7 +751,7 @@ LDEV void block_min2(double &a, double &b) {
 // calcCourantConstraintForElems / calcHydroConstraintForElems (2964-3054)
 template <bool COST>
 __global__ __launch_bounds__(BLOCK) void k_qeos(Batch B) {
-  LULESH_ELEM_PREAMBLE_G(B, QEOS_GROUP)
+  LULESH_ELEM_PREAMBLE_T(B, qeos_order)
   if (B.gated && !sc->active) return;  // uniform over the block
   const int quirks = B.quirks, keep_aux = B.keep_aux;
   double *__restrict__ dtpart = B.dtpart;
@@ -1096,16 +1086,13 @@ hipError_t launch_kernel(int id, hipStream_t st, const Batch &b) {
       hipLaunchKernelGGL(k_nodal, dim3(b.glaunch), dim3(BLOCK), 0, st, b);
       break;
     case KID_KIN:
-      hipLaunchKernelGGL(k_kin, dim3(grouped_grid(b.eblocks * (int)KIN_SUB, KIN_GROUP * KIN_SUB)),
-                         dim3(KIN_BLOCK), 0, st, b);
+      hipLaunchKernelGGL(k_kin, dim3(b.kin_launch), dim3(KIN_CHUNK), 0, st, b);
       break;
     case KID_QEOS:
       if (b.cost_mode)
-        hipLaunchKernelGGL(k_qeos<true>, dim3(grouped_grid(b.eblocks, QEOS_GROUP)), dim3(BLOCK), 0,
-                           st, b);
+        hipLaunchKernelGGL(k_qeos<true>, dim3(b.qeos_launch), dim3(BLOCK), 0, st, b);
       else
-        hipLaunchKernelGGL(k_qeos<false>, dim3(grouped_grid(b.eblocks, QEOS_GROUP)), dim3(BLOCK),
-                           0, st, b);
+        hipLaunchKernelGGL(k_qeos<false>, dim3(b.qeos_launch), dim3(BLOCK), 0, st, b);
       break;
     case KID_DT:
       hipLaunchKernelGGL(k_dt, dim3(b.n), dim3(1024), 0, st, b);

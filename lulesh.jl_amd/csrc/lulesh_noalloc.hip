@@ -61,10 +61,11 @@ size_t round_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
 // the scratch array (offsets in bytes, each 256-byte aligned)
 struct Carve {
   int64_t E, N;
-  int eb, gb, glaunch;
+  int eb, gb, glaunch, kin_launch, qeos_launch;
   NodalTile tile;
   int cz;
-  size_t off_nodes, off_temps, off_dtpart, off_sc, off_inst, off_eblk, off_gblk, off_gord, bytes;
+  size_t off_nodes, off_temps, off_dtpart, off_sc, off_inst, off_eblk, off_gblk, off_gord,
+      off_kin, off_qeos, bytes;
 };
 
 Carve carve(int nx, const DeviceScratch &d) {
@@ -80,6 +81,16 @@ Carve carve(int nx, const DeviceScratch &d) {
   // are cut at ceil(gb * x / 8) and its longest run has ceil(gb / 8) blocks
   const int runlen = (c.gb + 7) / 8;
   c.glaunch = runlen * 8 > 8 ? runlen * 8 : 8;
+  // kin_table for one instance (band rotation 0): XCD x runs band x of
+  // every plane, so the launch is 8 x nx x the widest band
+  const int nch = kin_chunks(nx);
+  int widest = 1;
+  for (int x = 0; x < 8; ++x) {
+    const int w = kin_band_lo(nch, x + 1) - kin_band_lo(nch, x);
+    widest = w > widest ? w : widest;
+  }
+  c.kin_launch = 8 * nx * widest;
+  c.qeos_launch = qeos_launch(c.eb);
   size_t o = 0;
   c.off_nodes = o;  o += round_up(6 * (size_t)c.N * sizeof(double));
   c.off_temps = o;  o += round_up(7 * (size_t)c.E * sizeof(double));
@@ -89,19 +100,21 @@ Carve carve(int nx, const DeviceScratch &d) {
   c.off_eblk = o;   o += round_up((size_t)c.eb * sizeof(int));
   c.off_gblk = o;   o += round_up((size_t)c.gb * sizeof(int));
   c.off_gord = o;   o += round_up((size_t)c.glaunch * sizeof(int));
+  c.off_kin = o;    o += round_up((size_t)c.kin_launch * sizeof(KinWork));
+  c.off_qeos = o;   o += round_up((size_t)c.qeos_launch * sizeof(int));
   c.bytes = o;
   return c;
 }
 
 // Writes everything the cycle's kernels read from the scratch array: the
 // instance descriptor, the scalars (error code clear, node-set parity 0,
-// k_nodal's finish count 0), the element-block table (all instance 0), the
-// k_nodal block table and its XCD dispatch order -- nodal_order for a single
-// instance: workgroup g goes to XCD r = g % 8 and runs the (g / 8)-th block
-// of that XCD's run [ceil(gb r / 8), ceil(gb (r+1) / 8)), or nothing.
+// k_nodal's finish count 0), the element-block table (all instance 0) and
+// the three dispatch tables, each as the host's planner builds it for a
+// single instance (nodal_order_one, kin_work_one, qeos_order_one).
 __global__ void k_noalloc_setup(InstDesc I, DevScalars S, InstDesc *inst, DevScalars *sc,
-                                int *eblock_inst, int *gblock_inst, int *gblock_order, int eb,
-                                int gb, int glaunch) {
+                                int *eblock_inst, int *gblock_inst, int *gblock_order,
+                                KinWork *kin_work, int *qeos_order, int eb, int gb, int glaunch,
+                                int kin_launch, int qeos_launch) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   if (t == 0) {
@@ -110,11 +123,9 @@ __global__ void k_noalloc_setup(InstDesc I, DevScalars S, InstDesc *inst, DevSca
   }
   for (int64_t i = t; i < eb; i += stride) eblock_inst[i] = 0;
   for (int64_t i = t; i < gb; i += stride) gblock_inst[i] = 0;
-  for (int64_t g = t; g < glaunch; g += stride) {
-    const int64_t r = g % 8, k = g / 8;
-    const int64_t lo = (gb * r + 7) / 8, hi = (gb * (r + 1) + 7) / 8;
-    gblock_order[g] = k < hi - lo ? (int)(lo + k) : -1;
-  }
+  for (int64_t g = t; g < glaunch; g += stride) gblock_order[g] = nodal_order_one(gb, g);
+  for (int64_t g = t; g < kin_launch; g += stride) kin_work[g] = kin_work_one(I.s, g);
+  for (int64_t g = t; g < qeos_launch; g += stride) qeos_order[g] = qeos_order_one(eb, g);
 }
 
 bool set_scalars(DevScalars &S, const double *v) {
@@ -209,6 +220,10 @@ int run(const lulesh_device_domain *dom, double *scalars, hipStream_t st, int de
   B.nblock_inst = nullptr;  // no node-shaped kernel runs in the cycle
   B.gblock_inst = (const int *)(base + c.off_gblk);
   B.gblock_order = (const int *)(base + c.off_gord);
+  B.kin_work = (const KinWork *)(base + c.off_kin);
+  B.qeos_order = (const int *)(base + c.off_qeos);
+  B.kin_launch = c.kin_launch;
+  B.qeos_launch = c.qeos_launch;
   B.eblocks = c.eb;
   B.nblocks = I.nbn;
   B.gblocks = c.gb;
@@ -242,11 +257,14 @@ int run(const lulesh_device_domain *dom, double *scalars, hipStream_t st, int de
   B.dt_base = 0;
   B.num_iters = 0;
 
-  const int work = c.eb > c.glaunch ? c.eb : c.glaunch;
+  int work = c.eb > c.glaunch ? c.eb : c.glaunch;
+  work = work > c.kin_launch ? work : c.kin_launch;
+  work = work > c.qeos_launch ? work : c.qeos_launch;
   const unsigned grid = (unsigned)((work + BLOCK - 1) / BLOCK);
   hipLaunchKernelGGL(k_noalloc_setup, dim3(grid), dim3(BLOCK), 0, st, I, S,
                      (InstDesc *)B.inst, B.sc, (int *)B.eblock_inst, (int *)B.gblock_inst,
-                     (int *)B.gblock_order, c.eb, c.gb, c.glaunch);
+                     (int *)B.gblock_order, (KinWork *)B.kin_work, (int *)B.qeos_order, c.eb,
+                     c.gb, c.glaunch, c.kin_launch, c.qeos_launch);
   if (hipGetLastError() != hipSuccess) return LULESH_ERR_HIP;
   const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
   for (int id : ids)

@@ -153,6 +153,138 @@ inline std::vector<int> nodal_order(const std::vector<int> &gblock_inst,
 }
 
 // ---------------------------------------------------------------------------
+// Work tables of the element kernels.  The dispatcher deals workgroup g to
+// XCD g % 8, and each XCD has its own L2.
+//
+// k_kin gathers the nodes of two node planes per element plane, so node
+// plane p+1 is read by element planes p and p+1.  Each element plane is cut
+// into KIN_CHUNK-element chunks and its chunks into eight contiguous bands;
+// XCD x runs band (x - base) mod 8 of every plane of an instance, plane
+// after plane, so the node rows an element plane shares with the next one
+// are read again by the same XCD while its L2 still holds them.  `base`
+// rotates the bands per instance to even out the XCDs' work in a batch.
+// ---------------------------------------------------------------------------
+constexpr int KIN_CHUNK = 64;
+struct KinWork {
+  int32_t inst;   // instance, -1: no work
+  int32_t start;  // first element of the chunk (instance-local); the chunk
+                  // ends at the next multiple of KIN_CHUNK or the plane end
+};
+inline int kin_chunks(int s) { return (int)(((int64_t)s * s + KIN_CHUNK - 1) / KIN_CHUNK); }
+// band b of a plane of nch chunks: chunks [kin_band_lo(nch, b), kin_band_lo(nch, b + 1))
+inline int kin_band_lo(int nch, int b) { return (int)(((int64_t)b * nch + 7) / 8); }
+
+//   s[i]      elements per edge of instance i,
+//   live      instances to list (null: all),
+//   base      band rotation per instance; an empty vector is filled in
+//             (greedy: each instance in turn takes the rotation that keeps
+//             the busiest XCD least busy, the first on ties),
+//   launch    workgroups launched: 0 sizes it (8 x the longest XCD
+//             sequence); a later call for fewer instances keeps it.
+// Returns the table: the chunk run by each launched workgroup.
+inline std::vector<KinWork> kin_table(const std::vector<int> &s, const std::vector<char> *live,
+                                      std::vector<int> &base, int &launch) {
+  const int n = (int)s.size();
+  if ((int)base.size() != n) {
+    base.assign((size_t)n, 0);
+    int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+      const int nch = kin_chunks(s[i]);
+      int best = 0;
+      int64_t best_max = -1;
+      for (int b = 0; b < 8; ++b) {
+        int64_t mx = 0;
+        for (int x = 0; x < 8; ++x) {
+          const int band = (x - b + 8) % 8;
+          const int64_t w = load[x] + (int64_t)s[i] * (kin_band_lo(nch, band + 1) - kin_band_lo(nch, band));
+          mx = w > mx ? w : mx;
+        }
+        if (best_max < 0 || mx < best_max) {
+          best_max = mx;
+          best = b;
+        }
+      }
+      base[(size_t)i] = best;
+      for (int x = 0; x < 8; ++x) {
+        const int band = (x - best + 8) % 8;
+        load[x] += (int64_t)s[i] * (kin_band_lo(nch, band + 1) - kin_band_lo(nch, band));
+      }
+    }
+  }
+  std::vector<KinWork> seq[8];
+  for (int i = 0; i < n; ++i) {
+    if (live && !(*live)[(size_t)i]) continue;
+    const int nch = kin_chunks(s[i]);
+    const int64_t plane = (int64_t)s[i] * s[i];
+    for (int x = 0; x < 8; ++x) {
+      const int band = (x - base[(size_t)i] + 8) % 8;
+      const int lo = kin_band_lo(nch, band), hi = kin_band_lo(nch, band + 1);
+      for (int p = 0; p < s[i]; ++p)
+        for (int ch = lo; ch < hi; ++ch)
+          seq[x].push_back(KinWork{i, (int32_t)(p * plane + (int64_t)ch * KIN_CHUNK)});
+    }
+  }
+  size_t len = 1;
+  for (int x = 0; x < 8; ++x) len = seq[x].size() > len ? seq[x].size() : len;
+  if (launch == 0) launch = (int)(8 * len);
+  std::vector<KinWork> table((size_t)launch, KinWork{-1, 0});
+  for (int x = 0; x < 8; ++x)
+    for (size_t j = 0; j < seq[x].size() && 8 * j + x < table.size(); ++j)
+      table[8 * j + x] = seq[x][j];
+  return table;
+}
+
+// k_qeos runs 256-element blocks (each writes one pair of dt partial
+// minima); XCD x takes runs of QEOS_GROUP consecutive blocks, the runs
+// dealt round-robin, so the eight XCDs sweep neighbouring addresses
+// together.  Blocks of instances not in `live` (null: all) are left out
+// (-1), as are the launch's padding workgroups.  launch = the block count
+// rounded up to a multiple of 8 x QEOS_GROUP.
+constexpr int QEOS_GROUP = 4;
+inline int qeos_launch(int eblocks) {
+  const int g = 8 * QEOS_GROUP;
+  return (eblocks + g - 1) / g * g;
+}
+inline std::vector<int> qeos_table(const std::vector<int> &eblock_inst, const std::vector<char> *live) {
+  const int nb = (int)eblock_inst.size();
+  std::vector<int> t((size_t)qeos_launch(nb), -1);
+  for (int g = 0; g < (int)t.size(); ++g) {
+    const int x = g % 8, k = g / 8;
+    const int v = ((k / QEOS_GROUP) * 8 + x) * QEOS_GROUP + k % QEOS_GROUP;
+    if (v < nb && (!live || (*live)[(size_t)eblock_inst[(size_t)v]])) t[(size_t)g] = v;
+  }
+  return t;
+}
+
+// The three tables of a single instance (all live, band rotation 0) in
+// closed form, entry g at a time: what the no-allocation entry point
+// writes on the device, equal to nodal_order / kin_table / qeos_table.
+#if defined(__HIPCC__)
+#define LULESH_HD __host__ __device__
+#else
+#define LULESH_HD
+#endif
+// nodal_order: every block weighs the same, so the XCD runs are cut at
+// ceil(gb r / 8)
+LULESH_HD inline int nodal_order_one(int64_t gb, int64_t g) {
+  const int64_t r = g % 8, k = g / 8;
+  const int64_t lo = (gb * r + 7) / 8, hi = (gb * (r + 1) + 7) / 8;
+  return k < hi - lo ? (int)(lo + k) : -1;
+}
+LULESH_HD inline KinWork kin_work_one(int64_t s, int64_t g) {
+  const int64_t nch = (s * s + KIN_CHUNK - 1) / KIN_CHUNK;
+  const int64_t x = g % 8, j = g / 8;
+  const int64_t lo = (x * nch + 7) / 8, hi = ((x + 1) * nch + 7) / 8, nb = hi - lo;
+  if (nb > 0 && j < s * nb) return KinWork{0, (int32_t)((j / nb) * s * s + (lo + j % nb) * KIN_CHUNK)};
+  return KinWork{-1, 0};
+}
+LULESH_HD inline int qeos_order_one(int64_t eb, int64_t g) {
+  const int64_t x = g % 8, k = g / 8;
+  const int64_t v = ((k / QEOS_GROUP) * 8 + x) * QEOS_GROUP + k % QEOS_GROUP;
+  return v < eb ? (int)v : -1;
+}
+
+// ---------------------------------------------------------------------------
 // binomial checkpointing: Griewank, "Achieving logarithmic growth of temporal
 // and spatial complexity in reverse automatic differentiation" (Optimization
 // Methods and Software 1, 1992) and Griewank & Walther, "Algorithm 799:

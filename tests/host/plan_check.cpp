@@ -11,11 +11,17 @@
 //   * nodal_order: every live block is dispatched exactly once, no finished
 //     instance's block is, and each XCD's run holds at most 1/8 of the work
 //     plus one block;
+//   * kin_table: every element of a live instance is in exactly one chunk,
+//     no chunk crosses a plane, and each band of an instance's planes is
+//     climbed by one XCD;
+//   * the single-instance closed forms of the three dispatch tables (what
+//     the no-allocation entry point writes on the device) equal the tables;
 //   * revolve_reverse: a recorded schedule reverses steps n-1..0 once each
 //     in that order, never needs more saved states than allowed, only
 //     advances from a saved state, and runs n + revolve_cost(n, c-1) steps;
 //   * build_regions: the product's arrays equal the oracle's.
 #include <cstdio>
+#include <algorithm>
 #include <cstdlib>
 #include <random>
 #include <set>
@@ -98,6 +104,75 @@ static void check_depth(std::mt19937 &rng) {
     CHECK(d >= 1 && d <= maxm);
     const int64_t cd = cost(d);
     for (int dd = 1; dd <= maxm; ++dd) CHECK(cd <= cost(dd));
+  }
+}
+
+// the single-instance closed forms the no-allocation entry point writes on
+// the device equal the planner's tables
+static void check_single_instance_tables() {
+  for (int s = 1; s <= 120; ++s) {
+    const NodalTile t = nodal_tile(s);
+    for (int cz : {1, 2, 5, s + 1}) {
+      const std::vector<int> gi((size_t)nodal_blocks(t, s, cz), 0), czv{cz};
+      int glaunch = 0;
+      const std::vector<int> ord = nodal_order(gi, czv, nullptr, glaunch);
+      const int gb = (int)gi.size(), runlen = (gb + 7) / 8;
+      CHECK(glaunch == (runlen * 8 > 8 ? runlen * 8 : 8));
+      for (int g = 0; g < glaunch; ++g) CHECK(ord[(size_t)g] == nodal_order_one(gb, g));
+    }
+    std::vector<int> base;
+    int launch = 0;
+    const std::vector<KinWork> kin = kin_table({s}, nullptr, base, launch);
+    CHECK(base.size() == 1 && base[0] == 0);
+    for (int g = 0; g < launch; ++g) {
+      const KinWork a = kin[(size_t)g], b = kin_work_one(s, g);
+      CHECK(a.inst == b.inst && (a.inst < 0 || a.start == b.start));
+    }
+    const int eb = (s * s * s + BLOCK - 1) / BLOCK;
+    const std::vector<int> q = qeos_table(std::vector<int>((size_t)eb, 0), nullptr);
+    CHECK((int)q.size() == qeos_launch(eb));
+    for (int g = 0; g < (int)q.size(); ++g) CHECK(q[(size_t)g] == qeos_order_one(eb, g));
+  }
+}
+
+// k_kin's table: every element of every live instance is in exactly one
+// chunk, a chunk never crosses a plane, and XCD x's chunks of an instance
+// all lie in one band of the plane (one XCD climbs each band)
+static void check_kin(std::mt19937 &rng) {
+  for (int trial = 0; trial < 60; ++trial) {
+    const int n = 1 + (int)(rng() % 12);
+    std::vector<int> s((size_t)n);
+    for (int i = 0; i < n; ++i) s[(size_t)i] = 1 + (int)(rng() % (trial < 5 ? 100 : 40));
+    std::vector<int> base;
+    int launch = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<char> live((size_t)n, 1);
+      if (pass)
+        for (int i = 0; i < n; ++i) live[(size_t)i] = (char)(rng() % 2);
+      const std::vector<KinWork> t = kin_table(s, pass ? &live : nullptr, base, launch);
+      CHECK((int)t.size() == launch && launch % 8 == 0);
+      std::vector<std::vector<int>> cover((size_t)n);
+      std::vector<std::vector<int>> xcd_of((size_t)n);
+      for (int i = 0; i < n; ++i) {
+        cover[(size_t)i].assign((size_t)s[(size_t)i] * s[(size_t)i] * s[(size_t)i], 0);
+        xcd_of[(size_t)i].assign((size_t)kin_chunks(s[(size_t)i]), -1);
+      }
+      for (int g = 0; g < launch; ++g) {
+        const KinWork w = t[(size_t)g];
+        if (w.inst < 0) continue;
+        CHECK(w.inst < n && live[(size_t)w.inst]);
+        const int64_t si = s[(size_t)w.inst], plane = si * si;
+        const int64_t p = w.start / plane, in = w.start % plane;
+        CHECK(in % KIN_CHUNK == 0 && p < si);
+        const int64_t end = std::min<int64_t>(w.start + KIN_CHUNK, (p + 1) * plane);
+        for (int64_t e = w.start; e < end; ++e) cover[(size_t)w.inst][(size_t)e] += 1;
+        int &x = xcd_of[(size_t)w.inst][(size_t)(in / KIN_CHUNK)];
+        CHECK(x < 0 || x == g % 8);
+        x = g % 8;
+      }
+      for (int i = 0; i < n; ++i)
+        for (int c : cover[(size_t)i]) CHECK(c == (live[(size_t)i] ? 1 : 0));
+    }
   }
 }
 
@@ -229,6 +304,8 @@ int main() {
   check_tiles();
   check_depth(rng);
   check_order(rng);
+  check_single_instance_tables();
+  check_kin(rng);
   check_revolve();
   check_regions();
   std::printf("failures %d\n", failures);
