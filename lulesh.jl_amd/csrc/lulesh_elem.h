@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "lulesh_cbrt.h"
+
 #define LDEV __device__ __forceinline__
 
 namespace lulesh {
@@ -36,60 +38,7 @@ LDEV double jmax(double x, double y) {
   return takey ? (isnan(x) ? x : y) : (isnan(y) ? y : x);
 }
 
-// ---------------------------------------------------------------------------
-// Correctly rounded cube root.  A float seed refined by two Newton steps
-// lands within a few ulp; the last bit is then decided exactly by comparing
-// 8*t*2^104 with the cubes of the two rounding midpoints (2K+-1)^3 in
-// 192-bit integer arithmetic (no exact midpoint exists for a cube root, so
-// the comparison is never a tie).
-// ---------------------------------------------------------------------------
-LDEV void cube54(uint64_t a, uint64_t w[3]) {
-  uint64_t sl = a * a, sh = __umul64hi(a, a);
-  uint64_t p0l = sl * a, p0h = __umul64hi(sl, a);
-  uint64_t p1l = sh * a, p1h = __umul64hi(sh, a);
-  uint64_t mid = p0h + p1l;
-  uint64_t carry = mid < p0h ? 1u : 0u;
-  w[0] = p0l;
-  w[1] = mid;
-  w[2] = p1h + carry;
-}
-LDEV int cmp192(const uint64_t a[3], const uint64_t b[3]) {
-  if (a[2] != b[2]) return a[2] < b[2] ? -1 : 1;
-  if (a[1] != b[1]) return a[1] < b[1] ? -1 : 1;
-  if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
-  return 0;
-}
-LDEV double cbrt_cr(double x) {
-  if (x == 0.0 || !isfinite(x)) return x + x;
-  double ax = fabs(x);
-  int sub = 0;
-  if (ax < 2.2250738585072014e-308) {  // subnormal: scale by 2^54, undo 2^18 later
-    ax = ax * 18014398509481984.0;
-    sub = 18;
-  }
-  uint64_t bits = (uint64_t)__double_as_longlong(ax);
-  int E = (int)((bits >> 52) & 0x7ff) - 1023;
-  uint64_t M = (bits & 0xfffffffffffffull) | 0x10000000000000ull;
-  int q = E >= 0 ? E / 3 : -((-E + 2) / 3);
-  int r = E - 3 * q;
-  uint64_t X = M << r;                         // t = X * 2^-52 in [1, 8)
-  double t = (double)X * 2.220446049250313080847e-16;
-  double y = (double)cbrtf((float)t);
-  y = y - (y * y * y - t) / (3.0 * y * y);
-  y = y - (y * y * y - t) / (3.0 * y * y);
-  int64_t K = (int64_t)(y * 4503599627370496.0 + 0.5);
-  uint64_t T8[3] = {0ull, X << 43, X >> 21};   // X * 2^107
-  for (int it = 0; it < 8; ++it) {
-    uint64_t c[3];
-    cube54((uint64_t)(2 * K + 1), c);
-    if (cmp192(c, T8) < 0) { ++K; continue; }
-    cube54((uint64_t)(2 * K - 1), c);
-    if (cmp192(c, T8) > 0) { --K; continue; }
-    break;
-  }
-  double res = ldexp((double)K, q - 52 - sub);
-  return signbit(x) ? -res : res;
-}
+// correctly rounded cube root: cbrt_cr (lulesh_cbrt.h)
 
 // ---------------------------------------------------------------------------
 // calcElemVolume, src/utils.jl:36-122

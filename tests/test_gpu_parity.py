@@ -370,6 +370,35 @@ def test_device_cbrt_is_correctly_rounded():
         assert g == _exact_cbrt_cr(float(v)), v
 
 
+def test_device_cbrt_many_and_near_midpoints():
+    """The device cube root (a fast decision with an exact 192-bit
+    fallback) against the oracle's exact one, bit for bit: 200,000 random
+    doubles over every exponent (subnormals included) and the committed
+    inputs whose cube roots lie within about 2^-20 ulp of a rounding
+    midpoint (tests/golden/cbrt_tight_cases.json), each scaled by 2^(3j)."""
+    import ctypes
+    import json
+    from lulesh_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(5)
+    raw = rng.integers(0, 2 ** 63, 200000, dtype=np.uint64)
+    expo = rng.integers(0, 2047, 200000, dtype=np.uint64)
+    expo[::53] = 0  # subnormals
+    raw = (raw & np.uint64(0x800FFFFFFFFFFFFF)) | (expo << np.uint64(52))
+    xs = raw.view(np.float64)
+    xs = xs[np.isfinite(xs)]
+    tight = [float.fromhex(h) for h in json.load(open(os.path.join(
+        os.path.dirname(__file__), "golden", "cbrt_tight_cases.json")))["inputs_hex"]]
+    xs = np.concatenate([xs] + [np.ldexp(np.array(tight), 3 * j) for j in (-300, -5, 0, 1, 7, 300)])
+    out = np.zeros_like(xs)
+    P = _lib.P_double
+    assert L.lulesh_math_check(0, xs.ctypes.data_as(P), xs.ctypes.data_as(P),
+                               out.ctypes.data_as(P), xs.size, 0) == 0
+    ref = np.array([O.cbrt(float(v)) for v in xs])
+    bad = np.nonzero(bits(out) != bits(ref))[0]
+    assert bad.size == 0, [(float(xs[i]).hex(), float(out[i]).hex(), float(ref[i]).hex()) for i in bad[:5]]
+
+
 def test_device_sqrt_and_division_are_ieee():
     from lulesh_amd import _lib
     L = _lib.load()
