@@ -173,18 +173,40 @@ def bench_batch_cycles(k, n=64, seed=0):
                       f"cycles 3-{2 + k}", "ms_per_cycle": round(wall * 1e3 / k, 4)}
 
 
-def cpu_baseline(nx, cycles, nthreads):
+STATE_FIELDS = ("e", "p", "q", "v", "ss", "x", "y", "z", "xd", "yd", "zd")
+STATE_SCALARS = ("time", "deltatime", "cycle", "dtcourant", "dthydro")
+
+
+def cpu_baseline(nx, cycles, nthreads, state=None, pin=None):
+    """The oracle (the CPU restatement of the reference) timed over `cycles`
+    cycles of the driver loop at s=nx: from `state` (fields and scalars of a
+    running domain) when given, else from the initial state.  With `pin` the
+    process runs on that one core for the measurement.  Returns the rate,
+    the seconds and the oracle's end state (e, x) for a cross-check."""
     import oracle as O
     od = O.OracleDomain(nx, O.QUIRKS_REFERENCE, nthreads)
+    if state is not None:
+        for name, val in state["fields"].items():
+            od.set_field(name, val)
+        for name, val in state["scalars"].items():
+            od.set(name, val)
     E = od.get("numElem")
-    t0 = time.perf_counter()
-    for _ in range(cycles):
-        rc = od.cycle_once()
-        if rc:
-            raise RuntimeError(f"oracle error {rc}")
-    dt = time.perf_counter() - t0
+    old = os.sched_getaffinity(0)
+    if pin is not None:
+        os.sched_setaffinity(0, {pin})
+    try:
+        t0 = time.perf_counter()
+        for _ in range(cycles):
+            rc = od.cycle_once()
+            if rc:
+                raise RuntimeError(f"oracle error {rc}")
+        dt = time.perf_counter() - t0
+    finally:
+        if pin is not None:
+            os.sched_setaffinity(0, old)
+    end = {"e": od.field("e"), "x": od.field("x"), "cycle": od.get("cycle")}
     od.close()
-    return E * cycles / dt, dt
+    return E * cycles / dt, dt, end
 
 
 def main():
@@ -311,15 +333,30 @@ def main():
 
     cpu = cpu_par = None
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        import numpy as np
         sys.path.insert(0, os.path.join(REPO, "oracle"))
-        v, secs = cpu_baseline(nx, args.cpu_cycles, args.cpu_threads)
+        # the sample starts where the timed GPU cycles ended (nodes moving,
+        # the blast under way), not at the initial state where nothing moves
+        c0 = dom.cycle
+        state = {"fields": {f: dom.get(f) for f in STATE_FIELDS},
+                 "scalars": {n: dom.get(n) for n in STATE_SCALARS}}
+        allowed = sorted(os.sched_getaffinity(0))
+        pin = allowed[0]
+        v, secs, end = cpu_baseline(nx, args.cpu_cycles, args.cpu_threads, state, pin)
+        # the same cycles on the GPU must end bit for bit where the oracle did
+        run(dom, c0 + args.cpu_cycles)
+        same = (dom.cycle == end["cycle"] and
+                np.array_equal(dom.get("e").view(np.uint64), end["e"].view(np.uint64)) and
+                np.array_equal(dom.get("x").view(np.uint64), end["x"].view(np.uint64)))
         cpu = {"value": round(v, 1), "unit": "zone-cycles/s", "cores": args.cpu_threads,
                "kind": "port",
-               "sample": f"CPU oracle (C++ restatement), s={nx}, cycles 1-{args.cpu_cycles} of the "
-                         f"same workload, {secs:.1f} s, {args.cpu_threads} thread(s)"}
+               "sample": f"CPU oracle (C++ restatement), s={nx}, cycles {c0 + 1}-{c0 + args.cpu_cycles} "
+                         f"of the same workload started from the GPU's state after cycle {c0}, "
+                         f"{secs:.1f} s, {args.cpu_threads} thread(s) pinned to core {pin}",
+               "sample_bitwise_equal_to_gpu": bool(same)}
         # the same sample over the host's cores (OpenMP; bit-identical results)
-        nthr = max(1, min(16, os.cpu_count() or 1))
-        vp, secs_p = cpu_baseline(nx, args.cpu_cycles, nthr)
+        nthr = max(1, min(16, len(allowed)))
+        vp, secs_p, _ = cpu_baseline(nx, args.cpu_cycles, nthr, state)
         cpu_par = {"value": round(vp, 1), "unit": "zone-cycles/s", "cores": nthr,
                    "sample": f"same sample, {secs_p:.1f} s, {nthr} OpenMP threads"}
 
