@@ -33,7 +33,23 @@ LHD double sq(double a) { return a * a; }
 
 // ---------------------------------------------------------------------------
 // geometry helpers (forward, no rounding contract: adjoints only)
+//
+// From here to the end of the hourglass adjoint the GPU build lets the
+// compiler fuse multiply-adds: this arithmetic has no branches, no rounding
+// contract and no bitwise partner on the forward side, and fusing removes a
+// third of a_force's floating-point instructions.  The limiter and EOS
+// adjoints below keep the unfused arithmetic of the forward kernels, whose
+// branch decisions they recompute.  Coordinates are read through X[n] (an
+// array, or on the GPU a view of values parked in LDS).
 // ---------------------------------------------------------------------------
+#ifdef __HIPCC__
+#pragma clang fp contract(fast)
+#endif
+
+// the volume derivative's 1/12 as a product (a division is ten dependent
+// instructions)
+constexpr double TWELFTH = 1.0 / 12.0;
+
 LHD double gamma_of(int n, int m) { return ((0xE0D35B68u >> (4 * n + m)) & 1u) ? -1.0 : 1.0; }
 
 LHD void cross(const double a[3], const double b[3], double r[3]) {
@@ -86,8 +102,8 @@ LHD int vd_slot(int call, bool q2) {
 
 // the volume derivative landing in slot `slot` (voluDer call `slot`,
 // except calls 1 and 3 swap slots without Q2)
-template <bool Q2>
-LHD void volume_derivative_slot(const double X[8], const double Y[8], const double Z[8], int slot,
+template <bool Q2, class XA>
+LHD void volume_derivative_slot(const XA &X, const XA &Y, const XA &Z, int slot,
                                 double &dx, double &dy, double &dz) {
   const int c = (!Q2 && slot == 1) ? 3 : ((!Q2 && slot == 3) ? 1 : slot);
   double x[6], y[6], z[6];
@@ -97,9 +113,9 @@ LHD void volume_derivative_slot(const double X[8], const double Y[8], const doub
     y[k] = Y[vd_node(c, k)];
     z[k] = Z[vd_node(c, k)];
   }
-  dx = vd_P(y, z) / 12.0;
-  dy = -vd_P(x, z) / 12.0;
-  dz = -vd_P(y, x) / 12.0;
+  dx = vd_P(y, z) * TWELFTH;
+  dy = -vd_P(x, z) * TWELFTH;
+  dz = -vd_P(y, x) * TWELFTH;
 }
 
 template <bool Q2>
@@ -115,9 +131,9 @@ LHD void volume_derivative_t(const double X[8], const double Y[8], const double 
       z[k] = Z[vd_node(c, k)];
     }
     const int sl = vd_slot(c, Q2);
-    dvdx[sl] = vd_P(y, z) / 12.0;
-    dvdy[sl] = -vd_P(x, z) / 12.0;
-    dvdz[sl] = -vd_P(y, x) / 12.0;
+    dvdx[sl] = vd_P(y, z) * TWELFTH;
+    dvdy[sl] = -vd_P(x, z) * TWELFTH;
+    dvdz[sl] = -vd_P(y, x) * TWELFTH;
   }
 }
 
@@ -137,9 +153,9 @@ LHD void volume_derivative_rev_t(const double X[8], const double Y[8], const dou
       z[k] = Z[vd_node(c, k)];
     }
     const int sl = vd_slot(c, Q2);
-    vd_P_rev(y, z, ldx[sl] / 12.0, ly, lz);
-    vd_P_rev(x, z, -ldy[sl] / 12.0, lx, lz);
-    vd_P_rev(y, x, -ldz[sl] / 12.0, ly, lx);
+    vd_P_rev(y, z, ldx[sl] * TWELFTH, ly, lz);
+    vd_P_rev(x, z, -ldy[sl] * TWELFTH, lx, lz);
+    vd_P_rev(y, x, -ldz[sl] * TWELFTH, ly, lx);
     LUNROLL
     for (int k = 0; k < 6; ++k) {
       lX[vd_node(c, k)] += lx[k];
@@ -256,8 +272,8 @@ struct HgArraySrcF {
 // adds a_y once more and subtracts a_z at node 1), so with S_f the sum of
 // lF over the nodes a_f reaches, lms = sum_f a_f . S_f and the adjoint of
 // a_f is ms * S_f.  Reads lF through src.lf(n, c); adds to lX..; returns lms.
-template <class Src>
-LHD double stress_rev_s(const double X[8], const double Y[8], const double Z[8], bool q1, double ms,
+template <class Src, class XA>
+LHD double stress_rev_s(const XA &X, const XA &Y, const XA &Z, bool q1, double ms,
                         const Src &src, double lX[8], double lY[8], double lZ[8]) {
   double lms = 0.0;
   LUNROLL
@@ -309,7 +325,8 @@ struct HgRev {
 
 // reverse of voluDer call c (the derivative landing in slot vd_slot(c)),
 // scaled by the slot's adjoint (lx, ly, lz); adds to lX, lY, lZ
-LHD void volume_derivative_call_rev(const double X[8], const double Y[8], const double Z[8], int c,
+template <class XA>
+LHD void volume_derivative_call_rev(const XA &X, const XA &Y, const XA &Z, int c,
                                     double lx, double ly, double lz, double lX[8], double lY[8],
                                     double lZ[8]) {
   double x[6], y[6], z[6], gx[6] = {0, 0, 0, 0, 0, 0}, gy[6] = {0, 0, 0, 0, 0, 0},
@@ -320,9 +337,9 @@ LHD void volume_derivative_call_rev(const double X[8], const double Y[8], const 
     y[k] = Y[vd_node(c, k)];
     z[k] = Z[vd_node(c, k)];
   }
-  vd_P_rev(y, z, lx / 12.0, gy, gz);
-  vd_P_rev(x, z, -ly / 12.0, gx, gz);
-  vd_P_rev(y, x, -lz / 12.0, gy, gx);
+  vd_P_rev(y, z, lx * TWELFTH, gy, gz);
+  vd_P_rev(x, z, -ly * TWELFTH, gx, gz);
+  vd_P_rev(y, x, -lz * TWELFTH, gy, gx);
   LUNROLL
   for (int k = 0; k < 6; ++k) {
     lX[vd_node(c, k)] += gx[k];
@@ -343,8 +360,8 @@ LHD void volume_derivative_call_rev(const double X[8], const double Y[8], const 
 // (src.v(c, n), src.lf(n, c)) and each lV[c][n] is final when formed, so it
 // goes straight to `sink.lv(c, n, value)`.  Adds the position adjoints to
 // lX, lY, lZ.
-template <bool Q2, class Src, class Sink>
-LHD HgRev hourglass_rev_s(const double X[8], const double Y[8], const double Z[8], const Src &src,
+template <bool Q2, class Src, class Sink, class XA>
+LHD HgRev hourglass_rev_s(const XA &X, const XA &Y, const XA &Z, const Src &src,
                           Sink &sink, double determ, double ss, double mass, double hourg,
                           double cbrt_determ, double lX[8], double lY[8], double lZ[8]) {
   const double volinv = 1.0 / determ;
@@ -454,6 +471,10 @@ LHD HgRev hourglass_rev(const double X[8], const double Y[8], const double Z[8],
     return hourglass_rev_s<true>(X, Y, Z, src, sink, determ, ss, mass, hourg, cbrt_determ, lX, lY, lZ);
   return hourglass_rev_s<false>(X, Y, Z, src, sink, determ, ss, mass, hourg, cbrt_determ, lX, lY, lZ);
 }
+
+#ifdef __HIPCC__
+#pragma clang fp contract(off)
+#endif
 
 // ---------------------------------------------------------------------------
 // calcMonotonicQGradientsForElems (src/domain.jl:2246-2317) adjoint.

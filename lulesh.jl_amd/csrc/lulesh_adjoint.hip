@@ -277,36 +277,61 @@ __global__ __launch_bounds__(BLOCK) void a_node1(AdjArgs A) {
 // ---- reverse of the corner forces (stress + FB hourglass) -------------------
 // velocities of S_c and force adjoints read from memory node by node, so
 // they do not occupy registers across the whole element
+// The element's node values are addressed by 32-bit byte offsets from
+// uniform array bases (the host keeps 8 N and 8 E below 2^32), so each
+// node costs one offset register shared by every array read there instead
+// of a 64-bit address per array.
+LDEV const double &at_off(const double *base, uint32_t off) {
+  return *(const double *)((const char *)base + off);
+}
 struct NodeSrc {
   const double *vx, *vy, *vz, *lF;
   int64_t N;
-  const int64_t *nid;
-  LDEV double v(int c, int n) const {
-    return (c == 0 ? vx : (c == 1 ? vy : vz))[nid[n]];
-  }
-  LDEV double lf(int n, int c) const { return lF[(int64_t)c * N + nid[n]]; }
+  uint32_t off[8];  // byte offsets of the element's nodes
+  LDEV double v(int c, int n) const { return at_off(c == 0 ? vx : (c == 1 ? vy : vz), off[n]); }
+  LDEV double lf(int n, int c) const { return at_off(lF + (int64_t)c * N, off[n]); }
 };
-// per-corner velocity adjoints straight to the corner buffer
+// per-corner adjoints straight to the corner buffer ([6][8][E])
 struct CornerSink {
   double *cr;
-  int64_t E, e;
-  LDEV void lv(int c, int n, double x) { cr[(int64_t)((3 + c) * 8 + n) * E + e] = x; }
+  int64_t E;
+  uint32_t off;  // byte offset of the element
+  LDEV void put(int k, int n, double x) {
+    *(double *)((char *)(cr + (int64_t)(k * 8 + n) * E) + off) = x;
+  }
+  LDEV void lv(int c, int n, double x) { put(3 + c, n, x); }
+};
+// One coordinate axis of the element's eight nodes, parked in LDS (stride
+// BLOCK, so a wave's reads are conflict-free) and re-read at every use: a
+// volatile LDS read cannot be merged with an earlier one, so the 24
+// coordinates hold no registers between uses.  That, with the 32-bit node
+// offsets, keeps a_force within 256 VGPRs: two waves per SIMD instead of one.
+struct LdsAxis {
+  const double *p;
+  __device__ double operator[](int n) const {
+    typedef __attribute__((address_space(3))) const volatile double lds_double;
+    return *(lds_double *)(p + n * BLOCK);
+  }
 };
 
 template <bool Q1, bool Q2>
-__global__ __launch_bounds__(BLOCK) void a_force(AdjArgs A) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void a_force(AdjArgs A) {
   const int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (e >= A.E) return;
   const DevFields &f = A.f;
   const NodeSet S = A.old;
   int64_t nid[8];
   elem_nodes_a(e, A.s, nid);
-  double X[8], Y[8], Z[8];
+  NodeSrc src{S.xd, S.yd, S.zd, A.lF, A.N, {}};
+  for (int n = 0; n < 8; ++n) src.off[n] = (uint32_t)nid[n] * 8u;
+  __shared__ double pos[3][8][BLOCK];
   for (int n = 0; n < 8; ++n) {
-    X[n] = S.x[nid[n]]; Y[n] = S.y[nid[n]]; Z[n] = S.z[nid[n]];
+    pos[0][n][threadIdx.x] = S.x[nid[n]];
+    pos[1][n][threadIdx.x] = S.y[nid[n]];
+    pos[2][n][threadIdx.x] = S.z[nid[n]];
   }
-  const NodeSrc src{S.xd, S.yd, S.zd, A.lF, A.N, nid};
-  CornerSink sink{A.corner_f, A.E, e};
+  const LdsAxis X{&pos[0][0][threadIdx.x]}, Y{&pos[1][0][threadIdx.x]}, Z{&pos[2][0][threadIdx.x]};
+  CornerSink sink{A.corner_f, A.E, (uint32_t)e * 8u};
   double lX[8] = {0}, lY[8] = {0}, lZ[8] = {0};
   const double hourg = A.sc->hgcoef;
   if (hourg > 0.0) {
@@ -326,11 +351,10 @@ __global__ __launch_bounds__(BLOCK) void a_force(AdjArgs A) {
   const double lms = lulesh_adj::stress_rev_s(X, Y, Z, Q1, ms, src, lX, lY, lZ);
   A.lp[e] += lms;
   A.lq[e] += lms;
-  double *cr = A.corner_f;
   for (int n = 0; n < 8; ++n) {
-    cr[(int64_t)(0 * 8 + n) * A.E + e] = lX[n];
-    cr[(int64_t)(1 * 8 + n) * A.E + e] = lY[n];
-    cr[(int64_t)(2 * 8 + n) * A.E + e] = lZ[n];
+    sink.put(0, n, lX[n]);
+    sink.put(1, n, lY[n]);
+    sink.put(2, n, lZ[n]);
   }
 }
 

@@ -736,6 +736,9 @@ int adj_setup_n(lulesh_ctx *c, int64_t ncycles, int64_t nstates, int64_t seg) {
   }
   if (seg <= 0) return 0;
   if (!B.ready) {
+    // a_force addresses node and element values by 32-bit byte offsets
+    if (8 * N > (int64_t)UINT32_MAX || 8 * E > (int64_t)UINT32_MAX)
+      return fail(c, LULESH_ERR_UNSUPPORTED, "the adjoint supports at most 2^29 - 1 nodes");
     AdjArgs &a = B.a;
     a.sc = c->b.sc;
     a.f = c->b.f;
@@ -812,9 +815,20 @@ double *slot_temps(lulesh_ctx *c, double *slot) {
   return slot + 6 * c->inst[0].N + 5 * c->inst[0].E;
 }
 
+// cycle arguments whose temporaries (vnew delv vdov delx_* delv_*) are
+// written straight into a slot, in the slot's order, instead of being
+// copied there after the cycle
+Batch into_slot(lulesh_ctx *c, Batch a, double *slot) {
+  const int64_t E = c->inst[0].E;
+  double *t = slot_temps(c, slot);
+  a.f.vnew = t; a.f.delv = t + E; a.f.vdov = t + 2 * E;
+  a.f.delx_xi = t + 3 * E; a.f.delx_eta = t + 4 * E; a.f.delx_zeta = t + 5 * E;
+  a.f.delv_xi = t + 6 * E; a.f.delv_eta = t + 7 * E; a.f.delv_zeta = t + 8 * E;
+  return a;
+}
+
 // the cycle after the scheduled dt was set (timeIncrement! replaced)
-int cycle_body(lulesh_ctx *c, bool constraints) {
-  const Batch a = make_args(c, 0, 0);
+int cycle_body(lulesh_ctx *c, bool constraints, const Batch &a) {
   const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
   for (int id : ids) {
     if (id == KID_DT && !constraints) break;
@@ -822,6 +836,9 @@ int cycle_body(lulesh_ctx *c, bool constraints) {
     if (rc) return rc;
   }
   return 0;
+}
+int cycle_body(lulesh_ctx *c, bool constraints) {
+  return cycle_body(c, constraints, make_args(c, 0, 0));
 }
 
 // adjoint kernel arguments reading cycle k from its stored slot (start
@@ -874,23 +891,29 @@ int grad_forward(lulesh_ctx *c, int64_t ncycles, long long cycle0, Save save, in
   Batch ta = a;  // timeIncrement! that also records the dt it takes
   ta.dt_record = B.dt;
   ta.dt_base = cycle0;
+  Batch da = ta;  // k_dt that also takes the next cycle's timeIncrement!
+  da.loop_mode = 2;
   int rc = clear_err(c);
+  // the first cycle's timeIncrement! on its own, every later one at the end
+  // of the previous cycle's k_dt (the copies between cycles read no scalar
+  // it changes)
+  if (!rc && ncycles > 0) rc = launch(c, KID_TIME_INCREMENT, ta);
   for (int64_t k = 0; k < ncycles && !rc; ++k) {
     double *buf = save(k);
     if (buf) rc = state_copy(c, buf, 1);
+    Batch ak = a, dk = k + 1 < ncycles ? da : a;
     if (!rc && k >= slot_from) {
-      const int64_t j = k - slot_from;
-      rc = slot_step(c, j ? slot_temps(c, seg_slot(c, j - 1)) : nullptr, seg_slot(c, j), nullptr, 0);
+      double *slot = seg_slot(c, k - slot_from);
+      rc = slot_step(c, nullptr, slot, nullptr, 0);
+      ak = into_slot(c, ak, slot);
+      dk = into_slot(c, dk, slot);
     }
-    if (!rc) rc = launch(c, KID_TIME_INCREMENT, ta);
-    const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS, KID_DT};
+    const int ids[] = {KID_NODAL, KID_KIN, KID_QEOS};
     for (int id : ids)
-      if (!rc) rc = launch(c, id, a);
+      if (!rc) rc = launch(c, id, ak);
+    if (!rc) rc = launch(c, KID_DT, dk);
   }
-  if (!rc && slot_from < ncycles) {
-    const int64_t L = ncycles - slot_from;
-    rc = slot_step(c, slot_temps(c, seg_slot(c, L - 1)), seg_slot(c, L), nullptr, 0);
-  }
+  if (!rc && slot_from < ncycles) rc = slot_step(c, nullptr, seg_slot(c, ncycles - slot_from), nullptr, 0);
   if (!rc) rc = finish(c, true);
   if (rc) return rc;
   if (dt_out) HIP_TRY(c, hipMemcpy(dt_out, B.dt, sizeof(double) * ncycles, hipMemcpyDeviceToHost));
@@ -967,13 +990,13 @@ int grad_run(lulesh_ctx *c, int64_t ncycles, int64_t every, double *grad_e, doub
         hipError_t e = launch_set_cycle(c->stream, c->b.sc, cycle0 + k0);
         if (e != hipSuccess) rc = hip_fail(c, e, "set cycle");
       }
-      // slot j <- S_{k0+j} (and the previous cycle's temporaries), dt, cycle
+      // slot j <- S_{k0+j}, dt, cycle; the cycle writes its temporaries there
+      const Batch a = make_args(c, 0, 0);
       for (int64_t j = 0; j < L && !rc; ++j) {
-        rc = slot_step(c, j ? slot_temps(c, seg_slot(c, j - 1)) : nullptr, seg_slot(c, j), B.dt,
-                       cycle0);
-        if (!rc) rc = cycle_body(c, false);
+        rc = slot_step(c, nullptr, seg_slot(c, j), B.dt, cycle0);
+        if (!rc) rc = cycle_body(c, false, into_slot(c, a, seg_slot(c, j)));
       }
-      if (!rc) rc = slot_step(c, slot_temps(c, seg_slot(c, L - 1)), seg_slot(c, L), nullptr, 0);
+      if (!rc) rc = slot_step(c, nullptr, seg_slot(c, L), nullptr, 0);
     }
     for (int64_t j = L - 1; j >= 0 && !rc; --j) {
       const int64_t k = k0 + j;

@@ -84,9 +84,9 @@ struct Batch {
   int quirks;
   int gated;                // kernels skip instances whose active flag is 0
   int keep_aux;             // also write fx.., xdd.., dxx.., ql, qq
-  int loop_mode;            // k_dt evaluates the driver-loop test and the next dt
+  int loop_mode;            // k_dt: 1 driver-loop test and next dt, 2 next dt always
   int flip;                 // k_nodal makes the node set it wrote current
-  double *dt_record;        // k_time_increment stores each instance-0 dt here
+  double *dt_record;        // k_time_increment (k_dt in loop_mode 2) stores each instance-0 dt here
   int64_t dt_base;          // ... at index cycle - 1 - dt_base (if non-null)
   int64_t num_iters;
   int cost_mode;            // k_qeos repeats each element's energy update

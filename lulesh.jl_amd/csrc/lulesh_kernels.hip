@@ -978,8 +978,10 @@ __global__ __launch_bounds__(BLOCK) void k_dtpart(Batch B) {
   }
 }
 
-// final minima; in driver-loop mode also the loop test of
-// examples/benchmark.jl:82,95 and the timeIncrement! of the next cycle
+// final minima; in driver-loop mode (loop_mode 1) also the loop test of
+// examples/benchmark.jl:82,95 and the timeIncrement! of the next cycle;
+// with loop_mode 2 the next cycle's timeIncrement! unconditionally, its dt
+// recorded as k_time_increment records it (a fixed number of cycles)
 __global__ __launch_bounds__(1024) void k_dt(Batch B) {
   const InstDesc I = B.inst[blockIdx.x];
   DevScalars *sc = B.sc + blockIdx.x;
@@ -1028,10 +1030,13 @@ __global__ __launch_bounds__(1024) void k_dt(Batch B) {
   }
   if (a < 1.0e+20) sc->dtcourant = a;
   if (b < 1.0e+20) sc->dthydro = b;
-  if (loop_mode) {
+  if (loop_mode == 1) {
     const int go = (sc->err == ERR_NONE) && (sc->cycle < sc->num_iters) && (sc->time < sc->stoptime);
     sc->active = go;
     if (go) time_increment(sc);
+  } else if (loop_mode == 2) {
+    time_increment(sc);
+    if (blockIdx.x == 0 && B.dt_record) B.dt_record[sc->cycle - 1 - B.dt_base] = sc->deltatime;
   }
 }
 
