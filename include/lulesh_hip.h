@@ -240,9 +240,9 @@ int lulesh_batch_verify(lulesh_ctx *ctx, int32_t inst, double out[4]);
  * The time step is held fixed at the recorded schedule (an input leaf, as
  * timeIncrement! is outside the differentiated call in the reference); the
  * context is left in the forward pass's end state.  One-domain contexts only
- * (LULESH_ERR_UNSUPPORTED otherwise), and every entry point of this section
- * needs fewer than 2^29 nodes (nx <= 811; its kernels address nodes and
- * elements by 32-bit byte offsets): LULESH_ERR_UNSUPPORTED above that.
+ * (LULESH_ERR_UNSUPPORTED otherwise).  The kernels of this section address
+ * nodes and elements by 32-bit byte offsets, which every accepted nx (<= 800)
+ * satisfies; the setup still checks it (LULESH_ERR_UNSUPPORTED).
  * lulesh_replay advances the state `ncycles` cycles with those time steps
  * instead of timeIncrement! (for finite-difference checks of the gradient). */
 int lulesh_grad_run(lulesh_ctx *ctx, int64_t ncycles, int64_t checkpoint_every, double *grad_e,
