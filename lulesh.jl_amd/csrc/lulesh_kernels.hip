@@ -82,6 +82,22 @@ LDEV T &at(T *base, uint32_t boff) {
 
 LDEV void raise(DevScalars *sc, int code) { atomicMin(&sc->err, code); }
 
+// Stores of k_kin's and k_qeos' outputs that do not keep their lines in the
+// caches: the next kernel reads them back from memory anyway, and with the
+// lines not left behind the following kernel's reads run faster (k_qeos
+// 229 -> 214 us at s=192 with k_kin's outputs so written, and k_kin 289 ->
+// 285 us with k_qeos' outputs; k_nodal's node stores so written make
+// k_nodal slower, 670 -> 685 us, and stay ordinary)
+template <typename T>
+LDEV void st_nt(T *base, uint32_t boff, T v) {
+  __builtin_nontemporal_store(v, (T *)((char *)base + boff));
+}
+template <typename T>
+LDEV void st_nt(T *p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
+
 // instance i's view of the flat field arrays
 LDEV DevFields slice(const DevFields &f, const InstDesc &I) {
   DevFields g;
@@ -637,9 +653,9 @@ __global__ __launch_bounds__(KIN_CHUNK) void k_kin(Batch B) {
   const double volume = elem_volume(X, Y, Z);
   const double vnew = volume / volo;
   if (vnew <= 0.0) raise(sc, ERR_KIN_VOLUME);
-  at(f.vnew, eo) = vnew;
-  at(f.delv, eo) = vnew - at(f.v, eo);
-  at(f.arealg, eo) = char_length(X, Y, Z, volume);
+  st_nt(f.vnew, eo, vnew);
+  st_nt(f.delv, eo, vnew - at(f.v, eo));
+  st_nt(f.arealg, eo, char_length(X, Y, Z, volume));
   {
     const double dt2 = 0.5 * sc->deltatime;
     double XS[8], YS[8], ZS[8], b[4][3];
@@ -655,12 +671,12 @@ __global__ __launch_bounds__(KIN_CHUNK) void k_kin(Batch B) {
     const double d2 = inv_detJ * vgrad_term(b, 1, YV);
     const double d3 = inv_detJ * vgrad_term(b, 2, ZV);
     const double vdov = (d1 + d2) + d3;
-    at(f.vdov, eo) = vdov;
+    st_nt(f.vdov, eo, vdov);
     if (keep_aux) {
       const double vdovthird = vdov / 3.0;
-      at(f.dxx, eo) = d1 - vdovthird;
-      at(f.dyy, eo) = d2 - vdovthird;
-      at(f.dzz, eo) = d3 - vdovthird;
+      st_nt(f.dxx, eo, d1 - vdovthird);
+      st_nt(f.dyy, eo, d2 - vdovthird);
+      st_nt(f.dzz, eo, d3 - vdovthird);
     }
     if (vnew <= 0.0) raise(sc, ERR_LAG_VOLUME);
   }
@@ -682,32 +698,32 @@ __global__ __launch_bounds__(KIN_CHUNK) void k_kin(Batch B) {
   ax = (dyi * dzj) - (dzi * dyj);
   ay = (dzi * dxj) - (dxi * dzj);
   az = (dxi * dyj) - (dyi * dxj);
-  at(f.delx_zeta, eo) = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  st_nt(f.delx_zeta, eo, vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny));
   ax = ax * norm; ay = ay * norm; az = az * norm;
   dxv = 0.25 * (sum4(XV[4], XV[5], XV[6], XV[7]) - sum4(XV[0], XV[1], XV[2], XV[3]));
   dyv = 0.25 * (sum4(YV[4], YV[5], YV[6], YV[7]) - sum4(YV[0], YV[1], YV[2], YV[3]));
   dzv = 0.25 * (sum4(ZV[4], ZV[5], ZV[6], ZV[7]) - sum4(ZV[0], ZV[1], ZV[2], ZV[3]));
-  at(f.delv_zeta, eo) = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+  st_nt(f.delv_zeta, eo, ((ax * dxv) + (ay * dyv)) + (az * dzv));
   // xi: j cross k
   ax = (dyj * dzk) - (dzj * dyk);
   ay = (dzj * dxk) - (dxj * dzk);
   az = (dxj * dyk) - (dyj * dxk);
-  at(f.delx_xi, eo) = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  st_nt(f.delx_xi, eo, vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny));
   ax = ax * norm; ay = ay * norm; az = az * norm;
   dxv = 0.25 * (sum4(XV[1], XV[2], XV[6], XV[5]) - sum4(XV[0], XV[3], XV[7], XV[4]));
   dyv = 0.25 * (sum4(YV[1], YV[2], YV[6], YV[5]) - sum4(YV[0], YV[3], YV[7], YV[4]));
   dzv = 0.25 * (sum4(ZV[1], ZV[2], ZV[6], ZV[5]) - sum4(ZV[0], ZV[3], ZV[7], ZV[4]));
-  at(f.delv_xi, eo) = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+  st_nt(f.delv_xi, eo, ((ax * dxv) + (ay * dyv)) + (az * dzv));
   // eta: k cross i
   ax = (dyk * dzi) - (dzk * dyi);
   ay = (dzk * dxi) - (dxk * dzi);
   az = (dxk * dyi) - (dyk * dxi);
-  at(f.delx_eta, eo) = vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny);
+  st_nt(f.delx_eta, eo, vol / sqrt((((ax * ax) + (ay * ay)) + (az * az)) + ptiny));
   ax = ax * norm; ay = ay * norm; az = az * norm;
   dxv = -0.25 * (sum4(XV[0], XV[1], XV[5], XV[4]) - sum4(XV[3], XV[2], XV[6], XV[7]));
   dyv = -0.25 * (sum4(YV[0], YV[1], YV[5], YV[4]) - sum4(YV[3], YV[2], YV[6], YV[7]));
   dzv = -0.25 * (sum4(ZV[0], ZV[1], ZV[5], ZV[4]) - sum4(ZV[3], ZV[2], ZV[6], ZV[7]));
-  at(f.delv_eta, eo) = ((ax * dxv) + (ay * dyv)) + (az * dzv);
+  st_nt(f.delv_eta, eo, ((ax * dxv) + (ay * dyv)) + (az * dzv));
 }
 
 // neighbour ids exactly as setupElementConnectivities stores them
@@ -826,8 +842,8 @@ __global__ __launch_bounds__(BLOCK) void k_qeos(Batch B) {
                ((delvxzeta * delvxzeta) * (1.0 - (phizeta * phizeta))));
     }
     if (keep_aux) {
-      f.qq[e] = qquad;
-      f.ql[e] = qlin;
+      st_nt(&f.qq[e], qquad);
+      st_nt(&f.ql[e], qlin);
     }
 
     // QStop on the q of the previous cycle
@@ -917,15 +933,15 @@ __global__ __launch_bounds__(BLOCK) void k_qeos(Batch B) {
     }
     double ss = ((pbvc * e_new) + (((vnewc * vnewc) * bvc) * p_new)) / rho0;
     ss = (ss <= TINY1) ? TINY3 : sqrt(ss);
-    f.p[e] = p_new;
-    f.e[e] = e_new;
-    f.q[e] = q_new;
-    f.ss[e] = ss;
+    st_nt(&f.p[e], p_new);
+    st_nt(&f.e[e], e_new);
+    st_nt(&f.q[e], q_new);
+    st_nt(&f.ss[e], ss);
 
     // updateVolumesForElems
     double tmpV = vnew;
     if (fabs(tmpV - 1.0) < sc->v_cut) tmpV = 1.0;
-    f.v[e] = tmpV;
+    st_nt(&f.v[e], tmpV);
     if (tmpV <= 0.0) raise(sc, ERR_UPDATE_VOLUME);
 
     // time-constraint terms; an element only counts where vdov != 0 and its
