@@ -103,7 +103,6 @@ LDEV void st_nt(T *p, T v) {
   __builtin_nontemporal_store(v, p);
 }
 
-
 // instance i's view of the flat field arrays
 LDEV DevFields slice(const DevFields &f, const InstDesc &I) {
   DevFields g;
