@@ -19,8 +19,16 @@
  * constructions; cycle 1 -- closed form; the whole quirk-free cycle -- the
  * LULESH 2.0 published s=30 result (932 iterations, e = 2.025075e+05);
  * quirk Q3 -- changes results and keeps the reference's non-termination at
- * s=30.  Parity unpinned at the last ulp: Julia's cbrt (not correctly
- * rounded) and its @simd reassociation of the <= 8-term corner gathers.
+ * s=30.  In tests/test_pins_monoq_verify.py: verify's symmetry numbers --
+ * hand-computed on an uploaded asymmetric plane; the MonoQ neighbour reads
+ * and quirk Q3 -- an exact-in-binary compression layer along each axis and
+ * side.  In tests/test_regions.py: the region index sets -- the nr = 1
+ * closed form, the invariants the algorithm fixes, hand-derived values of
+ * the partial sort, and the generator's seed (SHA-256 of the seed word).
+ * Parity unpinned at the last ulp: Julia's cbrt (not correctly rounded) and
+ * its @simd reassociation of the <= 8-term corner gathers.  Parity unpinned
+ * outright: the region draw stream for nr > 1 (no Julia run to compare
+ * with; it never reaches the arithmetic).
  *
  * Index conventions: every array here is 0-based in memory.  Index *values*
  * keep the reference's stored bases: nodelist, the six face-neighbour lists,
