@@ -519,6 +519,7 @@ int create(int32_t n, const int32_t *nx, const double *einit_scale, const double
   B.eblocks = eb;
   B.nblocks = nb;
   B.gblocks = 0;
+  B.nodal_lead = c->cus / 8 > 1 ? c->cus / 8 : 1;
   B.quirks = quirks;
   int rc = 0;
   hipError_t he = hipSetDevice(device);

@@ -76,6 +76,8 @@ struct Batch {
   const int *qeos_order;    // k_qeos block run by each launched workgroup, -1: none
   int eblocks, nblocks, gblocks;
   int glaunch;              // k_nodal workgroups launched (a multiple of 8)
+  int nodal_lead;           // CUs per XCD: k_nodal workgroups dealt to an XCD
+                            // before its CUs hold one each
   int kin_launch;           // k_kin workgroups launched
   int qeos_launch;          // k_qeos workgroups launched
   DevScalars *sc;           // [n]

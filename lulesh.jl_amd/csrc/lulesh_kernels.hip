@@ -392,8 +392,8 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
   const int oi = tx != 0, oj = ty != 0;
   const int iend = tx == I.tx - 1 ? s + 1 : i0 + etx - oi;
   const int jend = ty == I.ty - 1 ? s + 1 : j0 + ety - oj;
-  const int k0 = chunk * I.cz;
-  const int k1 = min(k0 + I.cz, s + 1);
+  const int k0 = nodal_chunk_begin(chunk, I.cz);
+  const int k1 = min(nodal_chunk_begin(chunk + 1, I.cz), s + 1);
   const int t = threadIdx.x, ti = t % etx, tj = t / etx;
   const int s1 = s + 1;
   // this thread's element in each plane, and its node column (threads past
@@ -425,8 +425,18 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
   double cs0 = 0.0, cs1 = 0.0, cs2 = 0.0, ch0 = 0.0, ch1 = 0.0, ch2 = 0.0;  // node plane ek
   double ns0 = 0.0, ns1 = 0.0, ns2 = 0.0, nh0 = 0.0, nh1 = 0.0, nh2 = 0.0;  // node plane ek+1
 
+  // Issue order.  A CU holds two k_nodal workgroups, one wave of each per
+  // SIMD, and a SIMD issues the older wave first when both are ready, so a
+  // workgroup that starts beside a running one falls behind it and marches
+  // its last planes alone, at the issue rate of one wave.  Such a workgroup
+  // (any after the first nodal_lead dealt to its XCD, one per CU) is issued
+  // first for the first half of its planes.
+  const int lead_end = k0 - 1 + (k1 - k0 + 1) / 2;
+  if ((int)(blockIdx.x >> 3) >= B.nodal_lead) __builtin_amdgcn_s_setprio(1);
+
   for (int ek = k0 - 1; ek < k1; ++ek) {
     const bool plane = ek >= 0 && ek < s;
+    if (ek == lead_end) __builtin_amdgcn_s_setprio(0);
     double hgf[8][3];
     if (plane && elem_col) {
       const int64_t e = ((int64_t)ek * s + ej) * s + ei;

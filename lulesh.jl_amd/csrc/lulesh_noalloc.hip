@@ -228,6 +228,7 @@ int run(const lulesh_device_domain *dom, double *scalars, hipStream_t st, int de
   B.nblocks = I.nbn;
   B.gblocks = c.gb;
   B.glaunch = c.glaunch;
+  B.nodal_lead = d.cus / 8 > 1 ? d.cus / 8 : 1;
   B.sc = (DevScalars *)(base + c.off_sc);
   DevFields &f = B.f;
   f.e = dom->e; f.p = dom->p; f.q = dom->q; f.v = dom->v; f.volo = dom->volo;

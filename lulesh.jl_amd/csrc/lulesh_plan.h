@@ -18,7 +18,7 @@ namespace lulesh {
 constexpr int BLOCK = 256;
 
 // k_nodal tiling: a workgroup marches a column of etx x ety element lanes
-// (etx * ety <= BLOCK) up `cz` node planes and owns the node columns whose
+// (etx * ety <= BLOCK) up one z-chunk (below) and owns the node columns whose
 // elements all lie in its column.  Along an axis of s elements (s+1 nodes),
 // with a lanes per tile: the first tile's elements start at node 0 and it
 // owns a nodes (no element precedes node 0); an inner tile's first element
@@ -51,8 +51,16 @@ inline NodalTile nodal_tile(int s, int lo = 16, int hi = 32) {
   }
   return best;
 }
+// z-chunks of a column: chunk c of depth cz owns node planes
+// [nodal_chunk_begin(c), nodal_chunk_begin(c + 1)) (the last one capped at
+// s + 1), and ceil((s + 1) / cz) chunks cover the column.  Node plane k
+// takes the corners of element planes k-1 and k, so a chunk marches one
+// element plane more than it owns node planes where there is one below
+// and above (the first chunk has none below, node plane s none above).
+constexpr int nodal_chunk_begin(int c, int cz) { return c * cz; }
+inline int nodal_chunks(int s, int cz) { return (s + cz) / cz; }
 inline int nodal_blocks(const NodalTile &t, int s, int cz) {
-  return t.tx * t.ty * ((s + 1 + cz - 1) / cz);
+  return t.tx * t.ty * nodal_chunks(s, cz);
 }
 
 // z-march depth of an instance: `depth` node planes per chunk, balanced so
@@ -73,8 +81,8 @@ struct NodalPlanInst {
 // `slots` (CUs x resident workgroups per CU): the estimate
 //   ceil(workgroups / slots) * (depth + 1)
 // is minimised over all depths (the smaller depth on ties, for balance).
-// At s=192 on 256 CUs x 2 this picks 3 chunks of 65 planes (507 workgroups,
-// one round) over 13 chunks of 15 (2197 workgroups, 4.3 rounds).
+// At s=192 on 256 CUs x 2 this picks 3 chunks of depth 65 (504 workgroups,
+// one round) over 13 chunks of depth 15 (2184 workgroups, 4.3 rounds).
 inline int auto_depth(const NodalPlanInst *insts, size_t n, int64_t slots) {
   int maxm = 2;
   for (size_t i = 0; i < n; ++i) maxm = insts[i].s + 1 > maxm ? insts[i].s + 1 : maxm;

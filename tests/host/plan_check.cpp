@@ -6,6 +6,9 @@
 // tests can run:
 //   * k_nodal tiles: along each axis the tiles own every node column once,
 //     and an owned column's elements are inside the tile's lanes;
+//   * z-chunks: for every depth the chunks own every node plane of a
+//     column once and each marches at most depth + 1 element planes;
+//     balanced_depth keeps the chunk count of the depth it is given;
 //   * auto_depth: its choice minimises the launch-round cost model (brute
 //     force over every depth);
 //   * nodal_order: every live block is dispatched exactly once, no finished
@@ -75,6 +78,32 @@ static void check_tiles() {
         check_axis(s, t.etx);
         check_axis(s, t.ety);
       }
+    }
+}
+
+// the node planes k_nodal's chunk c owns: [nodal_chunk_begin(c),
+// min(nodal_chunk_begin(c + 1), s + 1)); it marches element planes
+// max(k0 - 1, 0) .. min(k1, s) - 1
+static void check_chunks() {
+  for (int s = 1; s <= 260; ++s)
+    for (int cz = 1; cz <= s + 2; ++cz) {
+      const int n = nodal_chunks(s, cz);
+      CHECK(n >= 1);
+      std::vector<int> owner((size_t)s + 1, 0);
+      int widest = 0;
+      for (int c = 0; c < n; ++c) {
+        const int k0 = nodal_chunk_begin(c, cz);
+        const int k1 = std::min(nodal_chunk_begin(c + 1, cz), s + 1);
+        CHECK(k0 < k1);
+        for (int k = k0; k < k1 && k >= 0 && k <= s; ++k) owner[(size_t)k] += 1;
+        const int planes = std::min(k1, s) - std::max(k0 - 1, 0);
+        CHECK(planes >= 1 && planes <= cz + 1);
+        widest = std::max(widest, planes);
+      }
+      for (int k = 0; k <= s; ++k) CHECK(owner[(size_t)k] == 1);
+      CHECK(widest == (n == 1 ? s : n == 2 ? cz : cz + 1));
+      const int b = balanced_depth(s, cz);
+      CHECK(b >= 1 && b <= cz && b <= s + 1 && nodal_chunks(s, b) == n);
     }
 }
 
@@ -302,6 +331,7 @@ static void check_regions() {
 int main() {
   std::mt19937 rng(2024);
   check_tiles();
+  check_chunks();
   check_depth(rng);
   check_order(rng);
   check_single_instance_tables();
