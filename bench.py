@@ -315,7 +315,9 @@ def main():
     # them for k_nodal, whose corner forces stay in LDS
     roofline = {"bound": "hbm", "achieved": dk["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(dk["GB_s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": dominant, "footprint_GB_s": dk["footprint_GB_s"]}
+                "kernel": dominant, "footprint_GB_s": dk["footprint_GB_s"],
+                "basis": "algorithmic bytes of the unfused reference path (corner-force round "
+                         "trip included) per launch time; the kernel itself moves `traffic` bytes"}
 
     # k_nodal is bound by FP64 issue, not HBM: the same kernel against the
     # FP64 peak (its arithmetic is unfused mul/add for bitwise parity, so
