@@ -430,8 +430,9 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
   // workgroup that starts beside a running one falls behind it and marches
   // its last planes alone, at the issue rate of one wave.  Such a workgroup
   // (any after the first nodal_lead dealt to its XCD, one per CU) is issued
-  // first for the first half of its planes.
-  const int lead_end = k0 - 1 + (k1 - k0 + 1) / 2;
+  // first for the first 55 % of its planes (50 % leaves it ~20 us behind
+  // at s=192).
+  const int lead_end = k0 - 1 + (11 * (k1 - k0 + 1)) / 20;
   if ((int)(blockIdx.x >> 3) >= B.nodal_lead) __builtin_amdgcn_s_setprio(1);
 
   for (int ek = k0 - 1; ek < k1; ++ek) {
