@@ -110,6 +110,26 @@ def test_nodal_tile_shapes_bitwise(nx, cycles):
     dom.close()
 
 
+def test_nodal_depths_bitwise_nx96():
+    """One s=96 domain (6 x 7 tiles) with fixed z-march depths 2, 5 and 13:
+    2058, 840 and 336 k_nodal workgroups, i.e. several launch rounds, whose
+    later workgroups are issued first for part of their planes, and one
+    round.  Every depth ends bit-identical to the oracle."""
+    print("[depths] nx=96", flush=True)
+    od = O.OracleDomain(96, O.QUIRKS_REFERENCE, 8)
+    rc, _ = od.run(8)
+    assert rc == 0
+    for depth in (2, 5, 13):
+        print(f"[depths] nx=96 depth={depth}", flush=True)
+        dom = Domain(LuleshProblem(10, True, 96, 11, 1, 1))
+        dom.set_nodal_depth(depth)
+        assert dom.nodal_depth() == depth
+        assert run(dom, 8) == 8
+        assert_same(dom, od, STATE_E + STATE_N, f"nx=96 depth={depth}")
+        assert_scalars(dom, od, f"nx=96 depth={depth}")
+        dom.close()
+
+
 @pytest.mark.parametrize("nx", [8, 12])
 def test_run_to_stoptime(nx):
     """Driver loop to stoptime 1e-2 on the device vs the oracle's loop:
