@@ -308,13 +308,18 @@ int ensure_regions(lulesh_ctx *c) {
   return 0;
 }
 
+// z-chunk cut of the context's instances: balanced for a single domain;
+// a batch keeps the plain cut (the balanced one measured slower there).
+bool balanced_cut(const lulesh_ctx *c) { return c->n == 1; }
+
 // Default z-march depth for the context's instances (auto_depth, with a
 // round of CUs x resident k_nodal workgroups per CU).
 int auto_depth(const lulesh_ctx *c) {
   std::vector<NodalPlanInst> pi;
   pi.reserve(c->inst.size());
   for (const InstDesc &I : c->inst) pi.push_back({I.s, NodalTile{I.etx, I.ety, I.tx, I.ty}});
-  return lulesh::auto_depth(pi.data(), pi.size(), (int64_t)c->cus * c->nodal_per_cu);
+  return lulesh::auto_depth(pi.data(), pi.size(), (int64_t)c->cus * c->nodal_per_cu,
+                            balanced_cut(c));
 }
 
 // k_nodal dispatch table (nodal_order) uploaded for the instances in
@@ -389,9 +394,13 @@ int plan_nodal(lulesh_ctx *c, int cz) {
   int gb = 0;
   for (int i = 0; i < c->n; ++i) {
     InstDesc &I = c->inst[i];
-    I.cz = automatic ? balanced_depth(I.s, cz) : cz;
+    const NodalCut k = automatic ? depth_cut(I.s, cz, balanced_cut(c))
+                                 : nodal_cut(I.s, cz, balanced_cut(c));
+    I.cz = k.cz;
+    I.cz0 = k.first;
+    I.nch = k.chunks;
     I.gb0 = gb;
-    I.gbn = nodal_blocks(NodalTile{I.etx, I.ety, I.tx, I.ty}, I.s, I.cz);
+    I.gbn = nodal_blocks(NodalTile{I.etx, I.ety, I.tx, I.ty}, k);
     gb += I.gbn;
     gblock_inst.insert(gblock_inst.end(), I.gbn, i);
   }

@@ -61,6 +61,7 @@ struct InstDesc {
   int nb0, nbn;          // node-kernel blocks
   int gb0, gbn;          // k_nodal blocks
   int cz;                // k_nodal node planes per chunk (the z-march depth)
+  int cz0, nch;          // node planes of the first chunk, chunks per column
   double einit_scale;    // deposited energy = scale * the reference's einit
   double hgcoef;         // initial hgcoef
 };

@@ -392,8 +392,9 @@ __global__ __launch_bounds__(BLOCK) void k_nodal(Batch B) {
   const int oi = tx != 0, oj = ty != 0;
   const int iend = tx == I.tx - 1 ? s + 1 : i0 + etx - oi;
   const int jend = ty == I.ty - 1 ? s + 1 : j0 + ety - oj;
-  const int k0 = nodal_chunk_begin(chunk, I.cz);
-  const int k1 = min(nodal_chunk_begin(chunk + 1, I.cz), s + 1);
+  const NodalCut cut{I.cz0, I.cz, I.nch};
+  const int k0 = nodal_chunk_begin(cut, chunk);
+  const int k1 = nodal_chunk_end(cut, chunk, s);
   const int t = threadIdx.x, ti = t % etx, tj = t / etx;
   const int s1 = s + 1;
   // this thread's element in each plane, and its node column (threads past

@@ -63,7 +63,7 @@ struct Carve {
   int64_t E, N;
   int eb, gb, glaunch, kin_launch, qeos_launch;
   NodalTile tile;
-  int cz;
+  NodalCut cut;
   size_t off_nodes, off_temps, off_dtpart, off_sc, off_inst, off_eblk, off_gblk, off_gord,
       off_kin, off_qeos, bytes;
 };
@@ -75,8 +75,8 @@ Carve carve(int nx, const DeviceScratch &d) {
   c.eb = (int)((c.E + BLOCK - 1) / BLOCK);
   c.tile = nodal_tile(nx);
   const NodalPlanInst pi{nx, c.tile};
-  c.cz = balanced_depth(nx, auto_depth(&pi, 1, (int64_t)d.cus * d.nodal_per_cu));
-  c.gb = nodal_blocks(c.tile, nx, c.cz);
+  c.cut = depth_cut(nx, auto_depth(&pi, 1, (int64_t)d.cus * d.nodal_per_cu, true), true);
+  c.gb = nodal_blocks(c.tile, c.cut);
   // one instance: every block weighs the same, so nodal_order's XCD runs
   // are cut at ceil(gb * x / 8) and its longest run has ceil(gb / 8) blocks
   const int runlen = (c.gb + 7) / 8;
@@ -209,7 +209,9 @@ int run(const lulesh_device_domain *dom, double *scalars, hipStream_t st, int de
   I.nbn = (int)((c.N + BLOCK - 1) / BLOCK);
   I.gb0 = 0;
   I.gbn = c.gb;
-  I.cz = c.cz;
+  I.cz = c.cut.cz;
+  I.cz0 = c.cut.first;
+  I.nch = c.cut.chunks;
   I.einit_scale = 1.0;
   I.hgcoef = S.hgcoef;
 

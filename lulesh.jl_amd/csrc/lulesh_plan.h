@@ -2,7 +2,7 @@
 // CPU-only harness (built with the address and undefined-behaviour
 // sanitizers) compiles exactly this code:
 //   * the k_nodal tile of an instance (nodal_span, nodal_tile),
-//   * the z-march depth model (balanced_depth, auto_depth),
+//   * the z-chunk cuts and the depth model (nodal_cut, depth_cut, auto_depth),
 //   * the XCD-aware k_nodal dispatch table (nodal_order),
 //   * the binomial checkpoint schedule of the adjoint (revolve_cost,
 //     revolve_split, revolve_reverse).
@@ -51,24 +51,42 @@ inline NodalTile nodal_tile(int s, int lo = 16, int hi = 32) {
   }
   return best;
 }
-// z-chunks of a column: chunk c of depth cz owns node planes
-// [nodal_chunk_begin(c), nodal_chunk_begin(c + 1)) (the last one capped at
-// s + 1), and ceil((s + 1) / cz) chunks cover the column.  Node plane k
-// takes the corners of element planes k-1 and k, so a chunk marches one
-// element plane more than it owns node planes where there is one below
-// and above (the first chunk has none below, node plane s none above).
-constexpr int nodal_chunk_begin(int c, int cz) { return c * cz; }
-inline int nodal_chunks(int s, int cz) { return (s + cz) / cz; }
-inline int nodal_blocks(const NodalTile &t, int s, int cz) {
-  return t.tx * t.ty * nodal_chunks(s, cz);
+// z-chunks of a column.  Node plane k takes the corners of element planes
+// k-1 and k, so a chunk marches one element plane more than it owns node
+// planes, except where there is none below (node plane 0) or above (node
+// plane s).  A cut gives the first chunk `first` node planes and every later
+// one `cz` (the z-march depth), the last one what is left up to node plane
+// s; chunk c owns node planes [nodal_chunk_begin(c), nodal_chunk_end(c)).
+// Two cuts:
+//   plain     first = cz and ceil((s + 1) / cz) chunks: the first and the
+//             last chunk march one element plane fewer than the others;
+//   balanced  first = cz + 1 and ceil((s - 1) / cz) chunks (one if
+//             cz >= s - 1): every chunk but the last marches cz + 1 element
+//             planes, the last at most that.
+struct NodalCut {
+  int first, cz, chunks;
+};
+constexpr int nodal_chunk_begin(const NodalCut &k, int c) {
+  return c == 0 ? 0 : k.first + (c - 1) * k.cz;
 }
-
-// z-march depth of an instance: `depth` node planes per chunk, balanced so
-// that the chunks of a column are (nearly) equal
-inline int balanced_depth(int s, int depth) {
-  const int m = s + 1, chunks = (m + depth - 1) / depth;
-  return (m + chunks - 1) / chunks;
+constexpr int nodal_chunk_end(const NodalCut &k, int c, int s) {
+  return c + 1 >= k.chunks ? s + 1 : k.first + c * k.cz;
 }
+inline NodalCut nodal_cut(int s, int cz, bool balanced) {
+  if (!balanced) return NodalCut{cz, cz, (s + cz) / cz};
+  return NodalCut{cz + 1, cz, s > 1 && cz < s - 1 ? (s + cz - 2) / cz : 1};
+}
+// The cut for a requested `depth`: the plain cut's chunk count for it,
+// ceil((s + 1) / depth), with the depth lowered as far as that count allows
+// (nearly equal chunks).  The balanced cut of a depth never has more chunks
+// or a deeper chunk than the plain one.
+inline NodalCut depth_cut(int s, int depth, bool balanced) {
+  const int chunks = (s + depth) / depth;
+  if (!balanced) return nodal_cut(s, (s + chunks) / chunks, false);
+  const int cz = (s + 2 * chunks - 2) / chunks - 1;
+  return nodal_cut(s, cz > 1 ? cz : 1, true);
+}
+inline int nodal_blocks(const NodalTile &t, const NodalCut &k) { return t.tx * t.ty * k.chunks; }
 
 // one instance as the k_nodal planner sees it
 struct NodalPlanInst {
@@ -80,10 +98,12 @@ struct NodalPlanInst {
 // it marches (depth + 1 element planes), and the launch runs in rounds of
 // `slots` (CUs x resident workgroups per CU): the estimate
 //   ceil(workgroups / slots) * (depth + 1)
-// is minimised over all depths (the smaller depth on ties, for balance).
-// At s=192 on 256 CUs x 2 this picks 3 chunks of depth 65 (504 workgroups,
-// one round) over 13 chunks of depth 15 (2184 workgroups, 4.3 rounds).
-inline int auto_depth(const NodalPlanInst *insts, size_t n, int64_t slots) {
+// is minimised over all depths (the smaller depth on ties, for balance),
+// each instance cut by depth_cut.  At s=192 on 256 CUs x 2 this picks 3
+// chunks (504 workgroups, one round; plain cut 65, 66 and 63 element planes,
+// balanced 65, 65 and 64) over 13 chunks of depth 15 (2184 workgroups, 4.3
+// rounds).
+inline int auto_depth(const NodalPlanInst *insts, size_t n, int64_t slots, bool balanced) {
   int maxm = 2;
   for (size_t i = 0; i < n; ++i) maxm = insts[i].s + 1 > maxm ? insts[i].s + 1 : maxm;
   if (slots < 1) slots = 1;
@@ -94,9 +114,9 @@ inline int auto_depth(const NodalPlanInst *insts, size_t n, int64_t slots) {
     int deepest = 0;
     for (size_t i = 0; i < n; ++i) {
       const NodalPlanInst &I = insts[i];
-      const int cz = balanced_depth(I.s, d);
-      total += nodal_blocks(I.t, I.s, cz);
-      deepest = cz > deepest ? cz : deepest;
+      const NodalCut k = depth_cut(I.s, d, balanced);
+      total += nodal_blocks(I.t, k);
+      deepest = k.cz > deepest ? k.cz : deepest;
     }
     const int64_t cost = ((total + slots - 1) / slots) * (deepest + 1);
     if (best_cost < 0 || cost < best_cost) {

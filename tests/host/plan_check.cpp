@@ -6,9 +6,11 @@
 // tests can run:
 //   * k_nodal tiles: along each axis the tiles own every node column once,
 //     and an owned column's elements are inside the tile's lanes;
-//   * z-chunks: for every depth the chunks own every node plane of a
-//     column once and each marches at most depth + 1 element planes;
-//     balanced_depth keeps the chunk count of the depth it is given;
+//   * z-chunks: for every depth and both cuts the chunks own every node
+//     plane of a column once and each marches at most depth + 1 element
+//     planes (the balanced cut's all but the last exactly that); for a
+//     requested depth the balanced cut is never longer or deeper than the
+//     plain one;
 //   * auto_depth: its choice minimises the launch-round cost model (brute
 //     force over every depth);
 //   * nodal_order: every live block is dispatched exactly once, no finished
@@ -81,29 +83,40 @@ static void check_tiles() {
     }
 }
 
-// the node planes k_nodal's chunk c owns: [nodal_chunk_begin(c),
-// min(nodal_chunk_begin(c + 1), s + 1)); it marches element planes
-// max(k0 - 1, 0) .. min(k1, s) - 1
+// the node planes k_nodal's chunk c owns: [nodal_chunk_begin(k, c),
+// nodal_chunk_end(k, c, s)); it marches element planes
+// max(k0 - 1, 0) .. min(k1, s) - 1.  Returns the most any chunk marches.
+static int check_cut(int s, const NodalCut &k, bool balanced) {
+  CHECK(k.chunks >= 1 && k.cz >= 1);
+  std::vector<int> owner((size_t)s + 1, 0);
+  int widest = 0;
+  for (int c = 0; c < k.chunks; ++c) {
+    const int k0 = nodal_chunk_begin(k, c), k1 = nodal_chunk_end(k, c, s);
+    CHECK(k0 < k1);
+    for (int n = k0; n < k1 && n >= 0 && n <= s; ++n) owner[(size_t)n] += 1;
+    const int planes = std::min(k1, s) - std::max(k0 - 1, 0);
+    CHECK(planes >= 1 && planes <= k.cz + 1);
+    if (balanced && c + 1 < k.chunks) CHECK(planes == k.cz + 1);
+    widest = std::max(widest, planes);
+  }
+  for (int n = 0; n <= s; ++n) CHECK(owner[(size_t)n] == 1);
+  return widest;
+}
+
 static void check_chunks() {
   for (int s = 1; s <= 260; ++s)
     for (int cz = 1; cz <= s + 2; ++cz) {
-      const int n = nodal_chunks(s, cz);
-      CHECK(n >= 1);
-      std::vector<int> owner((size_t)s + 1, 0);
-      int widest = 0;
-      for (int c = 0; c < n; ++c) {
-        const int k0 = nodal_chunk_begin(c, cz);
-        const int k1 = std::min(nodal_chunk_begin(c + 1, cz), s + 1);
-        CHECK(k0 < k1);
-        for (int k = k0; k < k1 && k >= 0 && k <= s; ++k) owner[(size_t)k] += 1;
-        const int planes = std::min(k1, s) - std::max(k0 - 1, 0);
-        CHECK(planes >= 1 && planes <= cz + 1);
-        widest = std::max(widest, planes);
-      }
-      for (int k = 0; k <= s; ++k) CHECK(owner[(size_t)k] == 1);
-      CHECK(widest == (n == 1 ? s : n == 2 ? cz : cz + 1));
-      const int b = balanced_depth(s, cz);
-      CHECK(b >= 1 && b <= cz && b <= s + 1 && nodal_chunks(s, b) == n);
+      const NodalCut p = nodal_cut(s, cz, false), b = nodal_cut(s, cz, true);
+      CHECK(p.cz == cz && b.cz == cz);
+      const int n = p.chunks;
+      CHECK(check_cut(s, p, false) == (n == 1 ? s : n == 2 ? cz : cz + 1));
+      check_cut(s, b, true);
+      // a requested depth: the plain cut keeps its chunk count and lowers
+      // the depth; the balanced cut is never longer or deeper than it
+      const NodalCut dp = depth_cut(s, cz, false), db = depth_cut(s, cz, true);
+      CHECK(dp.cz >= 1 && dp.cz <= cz && dp.chunks == n);
+      CHECK(db.chunks <= dp.chunks);
+      CHECK(check_cut(s, db, true) <= check_cut(s, dp, false));
     }
 }
 
@@ -118,15 +131,16 @@ static void check_depth(std::mt19937 &rng) {
       maxm = s + 1 > maxm ? s + 1 : maxm;
     }
     const int64_t slots = 1 + (int64_t)(rng() % 1024);
-    const int d = auto_depth(in.data(), in.size(), slots);
+    const bool balanced = trial % 2 == 1;
+    const int d = auto_depth(in.data(), in.size(), slots, balanced);
     auto cost = [&](int dd) {
       int64_t total = 0;
       int deepest = 0;
       for (const auto &I : in) {
-        const int cz = balanced_depth(I.s, dd);
-        CHECK(cz >= 1 && cz <= I.s + 1);
-        total += nodal_blocks(I.t, I.s, cz);
-        deepest = cz > deepest ? cz : deepest;
+        const NodalCut k = depth_cut(I.s, dd, balanced);
+        CHECK(k.cz >= 1 && k.cz <= I.s + 1);
+        total += nodal_blocks(I.t, k);
+        deepest = k.cz > deepest ? k.cz : deepest;
       }
       return ((total + slots - 1) / slots) * (deepest + 1);
     };
@@ -142,7 +156,7 @@ static void check_single_instance_tables() {
   for (int s = 1; s <= 120; ++s) {
     const NodalTile t = nodal_tile(s);
     for (int cz : {1, 2, 5, s + 1}) {
-      const std::vector<int> gi((size_t)nodal_blocks(t, s, cz), 0), czv{cz};
+      const std::vector<int> gi((size_t)nodal_blocks(t, nodal_cut(s, cz, true)), 0), czv{cz};
       int glaunch = 0;
       const std::vector<int> ord = nodal_order(gi, czv, nullptr, glaunch);
       const int gb = (int)gi.size(), runlen = (gb + 7) / 8;
@@ -212,8 +226,9 @@ static void check_order(std::mt19937 &rng) {
     for (int i = 0; i < n; ++i) {
       const int s = 1 + (int)(rng() % 64);
       const NodalTile t = nodal_tile(s);
-      cz[i] = balanced_depth(s, 1 + (int)(rng() % 20));
-      gi.insert(gi.end(), nodal_blocks(t, s, cz[i]), i);
+      const NodalCut k = depth_cut(s, 1 + (int)(rng() % 20), false);
+      cz[i] = k.cz;
+      gi.insert(gi.end(), nodal_blocks(t, k), i);
     }
     int glaunch = 0;
     for (int pass = 0; pass < 3; ++pass) {

@@ -111,10 +111,11 @@ def test_nodal_tile_shapes_bitwise(nx, cycles):
 
 
 def test_nodal_depths_bitwise_nx96():
-    """One s=96 domain (6 x 7 tiles) with fixed z-march depths 2, 5 and 13:
-    2058, 840 and 336 k_nodal workgroups, i.e. several launch rounds, whose
-    later workgroups are issued first for part of their planes, and one
-    round.  Every depth ends bit-identical to the oracle."""
+    """One s=96 domain (6 x 7 tiles, balanced z-chunk cut) with fixed
+    z-march depths 2, 5 and 13: 2016, 798 and 336 k_nodal workgroups, i.e.
+    several launch rounds, whose later workgroups are issued first for part
+    of their planes, and one round.  Every depth ends bit-identical to the
+    oracle."""
     print("[depths] nx=96", flush=True)
     od = O.OracleDomain(96, O.QUIRKS_REFERENCE, 8)
     rc, _ = od.run(8)
