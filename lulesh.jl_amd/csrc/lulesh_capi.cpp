@@ -73,6 +73,7 @@ struct lulesh_ctx {
   int *ginst = nullptr;            // device table of gblock_host (regrown by plan_nodal)
   int64_t ginst_cap = 0;
   std::vector<int> gblock_host;  // instance of each k_nodal block
+  std::vector<int> gplanes_host; // element planes each k_nodal block marches
   int *gorder = nullptr;         // k_nodal dispatch table (regrown by plan_nodal)
   int64_t gorder_cap = 0;
   std::vector<int> gorder_host;
@@ -327,9 +328,7 @@ int auto_depth(const lulesh_ctx *c) {
 // share.  The first call for a plan (glaunch 0) sizes the launch.
 // Synchronous.
 int order_nodal(lulesh_ctx *c, const std::vector<char> *live) {
-  std::vector<int> cz(c->inst.size());
-  for (size_t i = 0; i < c->inst.size(); ++i) cz[i] = c->inst[i].cz;
-  c->gorder_host = nodal_order(c->gblock_host, cz, live, c->b.glaunch);
+  c->gorder_host = nodal_order(c->gblock_host, c->gplanes_host, live, c->b.glaunch);
   const std::vector<int> &order = c->gorder_host;
   c->nodal_ready = false;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -390,7 +389,7 @@ int plan_nodal(lulesh_ctx *c, int cz) {
   // unchanged to all instances
   const bool automatic = cz <= 0;
   if (automatic) cz = auto_depth(c);
-  std::vector<int> gblock_inst;
+  std::vector<int> gblock_inst, gplanes;
   int gb = 0;
   for (int i = 0; i < c->n; ++i) {
     InstDesc &I = c->inst[i];
@@ -403,9 +402,12 @@ int plan_nodal(lulesh_ctx *c, int cz) {
     I.gbn = nodal_blocks(NodalTile{I.etx, I.ety, I.tx, I.ty}, k);
     gb += I.gbn;
     gblock_inst.insert(gblock_inst.end(), I.gbn, i);
+    for (int ch = 0; ch < k.chunks; ++ch)
+      gplanes.insert(gplanes.end(), I.tx * I.ty, nodal_chunk_planes(k, ch, I.s));
   }
   c->b.gblocks = gb;
   c->gblock_host = std::move(gblock_inst);
+  c->gplanes_host = std::move(gplanes);
   c->b.glaunch = 0;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (gb > c->ginst_cap) {

@@ -77,8 +77,10 @@ Carve carve(int nx, const DeviceScratch &d) {
   const NodalPlanInst pi{nx, c.tile};
   c.cut = depth_cut(nx, auto_depth(&pi, 1, (int64_t)d.cus * d.nodal_per_cu, true), true);
   c.gb = nodal_blocks(c.tile, c.cut);
-  // one instance: every block weighs the same, so nodal_order's XCD runs
-  // are cut at ceil(gb * x / 8) and its longest run has ceil(gb / 8) blocks
+  // one instance, its blocks taken as equally heavy (under the balanced
+  // cut only the last chunk marches fewer planes): nodal_order's XCD runs
+  // are then cut at ceil(gb * x / 8) and its longest run has ceil(gb / 8)
+  // blocks
   const int runlen = (c.gb + 7) / 8;
   c.glaunch = runlen * 8 > 8 ? runlen * 8 : 8;
   // kin_table for one instance (band rotation 0): XCD x runs band x of

@@ -87,6 +87,11 @@ inline NodalCut depth_cut(int s, int depth, bool balanced) {
   return nodal_cut(s, cz > 1 ? cz : 1, true);
 }
 inline int nodal_blocks(const NodalTile &t, const NodalCut &k) { return t.tx * t.ty * k.chunks; }
+// the element planes chunk c marches: max(k0 - 1, 0) .. min(k1, s) - 1
+inline int nodal_chunk_planes(const NodalCut &k, int c, int s) {
+  const int k0 = nodal_chunk_begin(k, c), k1 = nodal_chunk_end(k, c, s);
+  return (k1 < s ? k1 : s) - (k0 > 1 ? k0 - 1 : 0);
+}
 
 // one instance as the k_nodal planner sees it
 struct NodalPlanInst {
@@ -132,18 +137,18 @@ inline int auto_depth(const NodalPlanInst *insts, size_t n, int64_t slots, bool 
 // own L2.  XCD x gets the x-th of eight contiguous runs of the block order
 // (instance, chunk, tile row, tile), cut at equal sums of marched element
 // planes: neighbouring tiles, which read each other's halo nodes, share an
-// L2, and every XCD gets the same work.  Within a run the deepest chunks go
-// first (stable, so an instance's blocks stay together): the shallow ones
-// fill the launch's last round.
+// L2, and every XCD gets the same work.  Within a run the blocks that march
+// the most planes go first (stable, so an instance's blocks stay together):
+// the shallow ones -- short chunks included -- fill the last round.
 //   gblock_inst[v]  instance of k_nodal block v,
-//   inst_cz[i]      z-march depth of instance i,
+//   planes[v]       element planes block v marches (nodal_chunk_planes),
 //   live            instances to list (null: all),
 //   glaunch         workgroups launched: 0 sizes it (a multiple of 8, at
 //                   least 8); a later call keeps it, and lists the live
 //                   blocks in plain order when they need more rounds.
 // Returns the table: block run by each launched workgroup, -1 for none.
 inline std::vector<int> nodal_order(const std::vector<int> &gblock_inst,
-                                    const std::vector<int> &inst_cz,
+                                    const std::vector<int> &planes,
                                     const std::vector<char> *live, int &glaunch) {
   std::vector<int> lv;
   lv.reserve(gblock_inst.size());
@@ -151,7 +156,7 @@ inline std::vector<int> nodal_order(const std::vector<int> &gblock_inst,
   for (int v = 0; v < (int)gblock_inst.size(); ++v)
     if (!live || (*live)[gblock_inst[v]]) {
       lv.push_back(v);
-      total += inst_cz[gblock_inst[v]] + 1;
+      total += planes[v];
     }
   const int nl = (int)lv.size();
   int cut[9];
@@ -159,7 +164,7 @@ inline std::vector<int> nodal_order(const std::vector<int> &gblock_inst,
   int x = 1;
   int64_t acc = 0;
   for (int k = 0; k < nl; ++k) {
-    acc += inst_cz[gblock_inst[lv[k]]] + 1;
+    acc += planes[lv[k]];
     while (x < 8 && acc * 8 >= total * x) cut[x++] = k + 1;
   }
   while (x <= 8) cut[x++] = nl;
@@ -169,9 +174,8 @@ inline std::vector<int> nodal_order(const std::vector<int> &gblock_inst,
   std::vector<int> order((size_t)glaunch, -1);
   if (runlen * 8 <= glaunch) {
     for (int r = 0; r < 8; ++r) {
-      std::stable_sort(lv.begin() + cut[r], lv.begin() + cut[r + 1], [&](int a, int b) {
-        return inst_cz[gblock_inst[a]] > inst_cz[gblock_inst[b]];
-      });
+      std::stable_sort(lv.begin() + cut[r], lv.begin() + cut[r + 1],
+                       [&](int a, int b) { return planes[a] > planes[b]; });
       for (int k = 0; k < cut[r + 1] - cut[r]; ++k) order[(size_t)k * 8 + r] = lv[cut[r] + k];
     }
   } else {

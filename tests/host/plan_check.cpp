@@ -20,7 +20,8 @@
 //     no chunk crosses a plane, and each band of an instance's planes is
 //     climbed by one XCD;
 //   * the single-instance closed forms of the three dispatch tables (what
-//     the no-allocation entry point writes on the device) equal the tables;
+//     the no-allocation entry point writes on the device) equal the tables
+//     (k_nodal's for blocks of equal weight);
 //   * revolve_reverse: a recorded schedule reverses steps n-1..0 once each
 //     in that order, never needs more saved states than allowed, only
 //     advances from a saved state, and runs n + revolve_cost(n, c-1) steps;
@@ -156,9 +157,10 @@ static void check_single_instance_tables() {
   for (int s = 1; s <= 120; ++s) {
     const NodalTile t = nodal_tile(s);
     for (int cz : {1, 2, 5, s + 1}) {
-      const std::vector<int> gi((size_t)nodal_blocks(t, nodal_cut(s, cz, true)), 0), czv{cz};
+      const std::vector<int> gi((size_t)nodal_blocks(t, nodal_cut(s, cz, true)), 0);
+      const std::vector<int> pl(gi.size(), cz + 1);
       int glaunch = 0;
-      const std::vector<int> ord = nodal_order(gi, czv, nullptr, glaunch);
+      const std::vector<int> ord = nodal_order(gi, pl, nullptr, glaunch);
       const int gb = (int)gi.size(), runlen = (gb + 7) / 8;
       CHECK(glaunch == (runlen * 8 > 8 ? runlen * 8 : 8));
       for (int g = 0; g < glaunch; ++g) CHECK(ord[(size_t)g] == nodal_order_one(gb, g));
@@ -222,27 +224,29 @@ static void check_kin(std::mt19937 &rng) {
 static void check_order(std::mt19937 &rng) {
   for (int trial = 0; trial < 300; ++trial) {
     const int n = 1 + (int)(rng() % 40);
-    std::vector<int> gi, cz(n);
+    std::vector<int> gi, pl;
     for (int i = 0; i < n; ++i) {
       const int s = 1 + (int)(rng() % 64);
       const NodalTile t = nodal_tile(s);
-      const NodalCut k = depth_cut(s, 1 + (int)(rng() % 20), false);
-      cz[i] = k.cz;
+      const NodalCut k = depth_cut(s, 1 + (int)(rng() % 20), trial % 2 == 1);
       gi.insert(gi.end(), nodal_blocks(t, k), i);
+      for (int ch = 0; ch < k.chunks; ++ch)
+        pl.insert(pl.end(), t.tx * t.ty, nodal_chunk_planes(k, ch, s));
     }
+    CHECK(pl.size() == gi.size());
     int glaunch = 0;
     for (int pass = 0; pass < 3; ++pass) {
       std::vector<char> live(n, 1);
       if (pass > 0)
         for (int i = 0; i < n; ++i) live[i] = (char)(rng() % 3 != 0);
-      const std::vector<int> ord = nodal_order(gi, cz, pass ? &live : nullptr, glaunch);
+      const std::vector<int> ord = nodal_order(gi, pl, pass ? &live : nullptr, glaunch);
       CHECK(glaunch >= 8 && glaunch % 8 == 0 && (int)ord.size() == glaunch);
       std::vector<int> seen(gi.size(), 0);
       int64_t total = 0, maxw = 0;
       for (size_t v = 0; v < gi.size(); ++v)
         if (live[gi[v]]) {
-          total += cz[gi[v]] + 1;
-          maxw = cz[gi[v]] + 1 > maxw ? cz[gi[v]] + 1 : maxw;
+          total += pl[v];
+          maxw = pl[v] > maxw ? pl[v] : maxw;
         }
       std::vector<int64_t> xcd(8, 0);
       for (int g = 0; g < glaunch; ++g) {
@@ -252,7 +256,7 @@ static void check_order(std::mt19937 &rng) {
         if (v >= (int)gi.size()) continue;
         seen[v] += 1;
         CHECK(live[gi[v]]);
-        xcd[g % 8] += cz[gi[v]] + 1;
+        xcd[g % 8] += pl[v];
       }
       for (size_t v = 0; v < gi.size(); ++v) CHECK(seen[v] == (live[gi[v]] ? 1 : 0));
       // the first plan cuts the block order into eight equal-work runs
