@@ -328,7 +328,8 @@ int auto_depth(const lulesh_ctx *c) {
 // share.  The first call for a plan (glaunch 0) sizes the launch.
 // Synchronous.
 int order_nodal(lulesh_ctx *c, const std::vector<char> *live) {
-  c->gorder_host = nodal_order(c->gblock_host, c->gplanes_host, live, c->b.glaunch);
+  c->gorder_host = nodal_order(c->gblock_host, c->gplanes_host, live, c->b.glaunch,
+                               (int64_t)c->cus * c->nodal_per_cu);
   const std::vector<int> &order = c->gorder_host;
   c->nodal_ready = false;
   HIP_TRY(c, hipStreamSynchronize(c->stream));

@@ -145,11 +145,17 @@ inline int auto_depth(const NodalPlanInst *insts, size_t n, int64_t slots, bool 
 //   live            instances to list (null: all),
 //   glaunch         workgroups launched: 0 sizes it (a multiple of 8, at
 //                   least 8); a later call keeps it, and lists the live
-//                   blocks in plain order when they need more rounds.
+//                   blocks in plain order when they need more rounds,
+//   slots           workgroups the GPU holds at once (0: unknown).  When
+//                   the live blocks fit in one round the runs are cut at
+//                   equal block counts instead: every block then has a
+//                   slot of its own from the start, and a run longer than
+//                   its XCD's share of the slots would need a second round.
 // Returns the table: block run by each launched workgroup, -1 for none.
 inline std::vector<int> nodal_order(const std::vector<int> &gblock_inst,
                                     const std::vector<int> &planes,
-                                    const std::vector<char> *live, int &glaunch) {
+                                    const std::vector<char> *live, int &glaunch,
+                                    int64_t slots = 0) {
   std::vector<int> lv;
   lv.reserve(gblock_inst.size());
   int64_t total = 0;
@@ -163,9 +169,13 @@ inline std::vector<int> nodal_order(const std::vector<int> &gblock_inst,
   cut[0] = 0;
   int x = 1;
   int64_t acc = 0;
-  for (int k = 0; k < nl; ++k) {
-    acc += planes[lv[k]];
-    while (x < 8 && acc * 8 >= total * x) cut[x++] = k + 1;
+  if (slots > 0 && nl <= slots) {
+    for (; x < 8; ++x) cut[x] = (int)(((int64_t)nl * x + 7) / 8);
+  } else {
+    for (int k = 0; k < nl; ++k) {
+      acc += planes[lv[k]];
+      while (x < 8 && acc * 8 >= total * x) cut[x++] = k + 1;
+    }
   }
   while (x <= 8) cut[x++] = nl;
   int runlen = 0;

@@ -239,7 +239,9 @@ static void check_order(std::mt19937 &rng) {
       std::vector<char> live(n, 1);
       if (pass > 0)
         for (int i = 0; i < n; ++i) live[i] = (char)(rng() % 3 != 0);
-      const std::vector<int> ord = nodal_order(gi, pl, pass ? &live : nullptr, glaunch);
+      // with the slot count known and one round enough, equal block counts
+      const int64_t slots = trial % 3 == 2 ? (int64_t)gi.size() + (int64_t)(rng() % 64) : 0;
+      const std::vector<int> ord = nodal_order(gi, pl, pass ? &live : nullptr, glaunch, slots);
       CHECK(glaunch >= 8 && glaunch % 8 == 0 && (int)ord.size() == glaunch);
       std::vector<int> seen(gi.size(), 0);
       int64_t total = 0, maxw = 0;
@@ -259,9 +261,20 @@ static void check_order(std::mt19937 &rng) {
         xcd[g % 8] += pl[v];
       }
       for (size_t v = 0; v < gi.size(); ++v) CHECK(seen[v] == (live[gi[v]] ? 1 : 0));
-      // the first plan cuts the block order into eight equal-work runs
-      if (pass == 0)
+      // the first plan cuts the block order into eight equal-work runs, or
+      // equal-count ones when the slots hold every block
+      if (pass == 0 && slots == 0)
         for (int x = 0; x < 8; ++x) CHECK(xcd[x] * 8 <= total + 8 * maxw);
+      if (pass == 0 && slots > 0) {
+        int64_t nl = 0;
+        std::vector<int64_t> cnt(8, 0);
+        for (int g = 0; g < glaunch; ++g)
+          if (ord[g] >= 0) {
+            cnt[g % 8] += 1;
+            nl += 1;
+          }
+        for (int x = 0; x < 8; ++x) CHECK(cnt[x] * 8 >= nl - 7 && cnt[x] * 8 <= nl + 7);
+      }
     }
   }
 }
