@@ -352,7 +352,9 @@ __global__ void k_run_begin(Batch B) {
 // lagrangeNodal (src/domain.jl:1840-1872) as one kernel.
 //
 // A workgroup owns a column of nodes (up to etx x ety; 16 x 14 for inner
-// tiles at s=192, see nodal_tile) and a chunk of cz node planes.  It marches
+// tiles at s=192, see nodal_tile) and a z-chunk of its node planes (the
+// instance's NodalCut: cz0 planes for the first chunk, cz for the others,
+// the last one up to node plane s).  It marches
 // up the chunk one element plane at a time: its threads compute the corner
 // forces of the etx x ety elements around the node tile
 // (integrateStressForElems + calcFBHourglassForceForElems arithmetic), stage
