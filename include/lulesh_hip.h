@@ -294,7 +294,9 @@ int lulesh_shadow_download(lulesh_ctx *ctx, const char *field, double *dst, int6
 #define LULESH_K_COUNT 4
 int lulesh_set_timing(lulesh_ctx *ctx, int32_t on);
 /* k_nodal geometry (no reference counterpart): each workgroup marches
- * `depth` node planes of its node column (16x14 nodes for large domains).
+ * `depth` node planes of its node column (16x14 nodes for large domains);
+ * in a single-domain context the first chunk takes one plane more, so
+ * that every chunk but the last marches depth + 1 element planes.
  * 0 restores the default: per instance, node planes split into equal
  * chunks whose depth minimises (launch rounds x planes per workgroup), a
  * round being CUs x the workgroups one CU holds.  Tile shape: per
