@@ -253,26 +253,41 @@ inline std::vector<KinWork> kin_table(const std::vector<int> &s, const std::vect
       }
     }
   }
-  std::vector<KinWork> seq[8];
-  for (int i = 0; i < n; ++i) {
-    if (live && !(*live)[(size_t)i]) continue;
+  // XCD x's sequence: the live instances in order, each band x's chunks
+  // plane by plane; entry j of it is workgroup 8 j + x.  Counted first, then
+  // written in place (the driver loop rebuilds the table whenever an
+  // instance finishes, while the GPU waits).
+  auto band_of = [&](int i, int x, int &lo, int &hi) {
     const int nch = kin_chunks(s[i]);
-    const int64_t plane = (int64_t)s[i] * s[i];
-    for (int x = 0; x < 8; ++x) {
-      const int band = (x - base[(size_t)i] + 8) % 8;
-      const int lo = kin_band_lo(nch, band), hi = kin_band_lo(nch, band + 1);
-      for (int p = 0; p < s[i]; ++p)
-        for (int ch = lo; ch < hi; ++ch)
-          seq[x].push_back(KinWork{i, (int32_t)(p * plane + (int64_t)ch * KIN_CHUNK)});
+    const int band = (x - base[(size_t)i] + 8) % 8;
+    lo = kin_band_lo(nch, band);
+    hi = kin_band_lo(nch, band + 1);
+  };
+  int64_t len = 1;
+  for (int x = 0; x < 8; ++x) {
+    int64_t m = 0;
+    for (int i = 0; i < n; ++i) {
+      if (live && !(*live)[(size_t)i]) continue;
+      int lo, hi;
+      band_of(i, x, lo, hi);
+      m += (int64_t)s[i] * (hi - lo);
     }
+    len = m > len ? m : len;
   }
-  size_t len = 1;
-  for (int x = 0; x < 8; ++x) len = seq[x].size() > len ? seq[x].size() : len;
   if (launch == 0) launch = (int)(8 * len);
   std::vector<KinWork> table((size_t)launch, KinWork{-1, 0});
-  for (int x = 0; x < 8; ++x)
-    for (size_t j = 0; j < seq[x].size() && 8 * j + x < table.size(); ++j)
-      table[8 * j + x] = seq[x][j];
+  for (int x = 0; x < 8; ++x) {
+    size_t g = (size_t)x;
+    for (int i = 0; i < n && g < table.size(); ++i) {
+      if (live && !(*live)[(size_t)i]) continue;
+      const int64_t plane = (int64_t)s[i] * s[i];
+      int lo, hi;
+      band_of(i, x, lo, hi);
+      for (int p = 0; p < s[i] && g < table.size(); ++p)
+        for (int ch = lo; ch < hi && g < table.size(); ++ch, g += 8)
+          table[g] = KinWork{i, (int32_t)(p * plane + (int64_t)ch * KIN_CHUNK)};
+    }
+  }
   return table;
 }
 
